@@ -1,0 +1,289 @@
+"""Lattice-level checks that pin the oracle to what the mathematics fixes.
+
+- the E-field force equals the gradient of the magnetic energy (finite differences),
+- leapfrog conserves energy to O(dt^2) and the Gauss law exactly,
+- energy, E.B and Gauss violation are gauge invariant, and leapfrog is gauge covariant,
+- the improved E.B converges to the continuum integral as O(a^4), the clover one as O(a^2),
+- Langevin noise/damping give equipartition <E^a E^a> = n_E T per link,
+- gradient flow descends the magnetic energy along the force,
+- the geodesic interpolation reaches its end points and the Simpson Delta N_CS
+  matches a fine quadrature of N_CS-dot along the straight path.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+
+
+def gauge_transform(lat, G):
+    """U_mu(x) -> G(x) U_mu(x) G(x+mu)^+, E_mu(x) -> G(x) E_mu(x) G(x)^+"""
+    nc, N = lat.nc, lat.N
+    s, n = O.su(nc), O.ne(nc)
+    U = lat.U.reshape(N[2], N[1], N[0], 3, s)
+    E = lat.E.reshape(N[2], N[1], N[0], 3, n)
+    Gr = G.reshape(N[2], N[1], N[0], s)
+    U2, E2 = np.zeros_like(U), np.zeros_like(E)
+    for z in range(N[2]):
+        for y in range(N[1]):
+            for x in range(N[0]):
+                g = Gr[z, y, x]
+                nb = [Gr[z, y, (x + 1) % N[0]], Gr[z, (y + 1) % N[1], x], Gr[(z + 1) % N[2], y, x]]
+                for mu in range(3):
+                    U2[z, y, x, mu] = O.mul(nc, O.mul(nc, g, U[z, y, x, mu]), nb[mu], "nd")
+                    E2[z, y, x, mu] = O.adj_act(nc, g, E[z, y, x, mu])
+    out = lat.copy()
+    out.U[:] = U2.ravel()
+    out.E[:] = E2.ravel()
+    return out
+
+
+def random_gauge(nc, V, seed):
+    r = np.random.default_rng(seed)
+    return np.concatenate([O.expi(nc, 1.0, r.normal(0, 1.0, O.ne(nc))) for _ in range(V)])
+
+
+def magnetic_energy(lat):
+    """H_B = (1/2) sum_x sum_mu B_mu^2 with B^2 = 4 ReTr(1-P)"""
+    return 0.5 * lat.V * float(lat.bulk()[3:].sum())
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_force_is_gradient_of_magnetic_energy(nc):
+    lat = O.Lattice(nc, 3)
+    lat.random(11, 0.6, 0.0, restore=False)
+    lat.E[:] = 0
+    dt = 1e-3
+    ref = lat.copy()
+    lat.leapfrog(1, dt)
+    rate = lat.E.reshape(-1, O.ne(nc)) / dt        # dE/dt at t=0
+    rng = np.random.default_rng(5)
+    links = rng.choice(3 * lat.V, size=6, replace=False)
+    eps = 1e-5
+    for l in links:
+        for a in range(O.ne(nc)):
+            vals = []
+            for sgn in (+1, -1):
+                t = ref.copy()
+                al = np.zeros(O.ne(nc))
+                al[a] = sgn * eps
+                u = t.U.reshape(-1, O.su(nc))
+                u[l] = O.mul(nc, O.expi(nc, 1.0, al), u[l])
+                vals.append(magnetic_energy(t))
+            grad = (vals[0] - vals[1]) / (2 * eps)
+            assert abs(rate[l, a] - grad) < 2e-8 * max(1.0, abs(grad)), (l, a, rate[l, a], grad)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_leapfrog_energy_drift_is_second_order(nc):
+    lat0 = O.Lattice(nc, 4)
+    lat0.random(21, 0.5, 0.4)
+    # E lives at half-integer times: E_{n-1/2} -> E_{n+1/2} with U_n, then
+    # U_n -> U_{n+1} (DYNAMICS/Dynamics.cpp:204-218); the synchronised energy
+    # 1/2 |(E_{n-1/2}+E_{n+1/2})/2|^2 + H_B(U_n) is conserved to O(dt^2)
+    drifts = []
+    for dt in (0.04, 0.02):
+        lat = lat0.copy()
+        dev, H0 = 0.0, None
+        for _ in range(int(round(1.6 / dt))):
+            Eprev, Uprev = lat.E.copy(), lat.copy()
+            lat.leapfrog(1, dt)
+            Esync = 0.5 * (Eprev + lat.E)
+            H = 0.5 * float(np.dot(Esync, Esync)) + magnetic_energy(Uprev)
+            H0 = H if H0 is None else H0
+            dev = max(dev, abs(H - H0))
+        drifts.append(dev)
+    ratio = drifts[0] / drifts[1]
+    assert 3.2 < ratio < 4.8, (drifts, ratio)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_leapfrog_conserves_gauss_law(nc):
+    lat = O.Lattice(nc, 4)
+    it, fm = lat.random(31, 0.4, 0.3)
+    assert fm <= 1e-14 and it < 10000
+    g0 = lat.gauss()
+    lat.leapfrog(25, 0.05)
+    g1 = lat.gauss()
+    assert g1[1] < 50 * max(g0[1], 1e-16) + 1e-13, (g0, g1)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_gauge_invariance_and_covariance(nc):
+    lat = O.Lattice(nc, (4, 5, 4))
+    lat.random(41, 0.7, 0.5, restore=False)
+    G = random_gauge(nc, lat.V, 43)
+    gt = gauge_transform(lat, G)
+    assert abs(gt.energy() - lat.energy()) < 1e-12 * lat.energy()
+    e1, e2 = lat.edotb(), gt.edotb()
+    assert abs(e1 - e2) < 1e-12 * max(1, abs(e1))
+    u1, u2 = lat.edotb_unimproved(), gt.edotb_unimproved()
+    assert abs(u1 - u2) < 1e-12 * max(1, abs(u1))
+    np.testing.assert_allclose(gt.edotb_map(), lat.edotb_map(), atol=1e-12)
+    assert abs(gt.gauss()[1] - lat.gauss()[1]) < 1e-12
+    # leapfrog commutes with the gauge transformation
+    a, b = lat.copy(), gt.copy()
+    a.leapfrog(3, 0.05)
+    b.leapfrog(3, 0.05)
+    ga = gauge_transform(a, G)
+    np.testing.assert_allclose(b.U, ga.U, atol=1e-12)
+    np.testing.assert_allclose(b.E, ga.E, atol=1e-12)
+
+
+def abelian_config(nc, N, b=1e-3, e=0.7):
+    """A_y = b sin(kx) cos(qz) along the third generator, E_z = e cos(kx) cos(qz).
+
+    Link variables carry line integrals: U_y = exp(i t^3 int A_y dy) and the
+    field conjugate to it is the line integral of E along the link, so the
+    z-link at (x, y, z) holds e cos(kx) (sin(q(z+1)) - sin(qz))/q.
+    """
+    lat = O.Lattice(nc, N)
+    k = q = 2 * np.pi / N
+    s, n = O.su(nc), O.ne(nc)
+    U = lat.U.reshape(N, N, N, 3, s)
+    E = lat.E.reshape(N, N, N, 3, n)
+    for z in range(N):
+        for x in range(N):
+            al = np.zeros(n)
+            al[2] = b * np.sin(k * x) * np.cos(q * z)
+            U[z, :, x, 1] = O.expi(nc, 1.0, al)
+            E[z, :, x, 2, 2] = e * np.cos(k * x) * (np.sin(q * (z + 1)) - np.sin(q * z)) / q
+    cont = e * b * k * N ** 3 / 4.0
+    return lat, cont
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_improved_edotb_is_fourth_order_in_a(nc):
+    err_i, err_u, sign = [], [], []
+    for N in (8, 16, 32):
+        lat, cont = abelian_config(nc, N)
+        ri, ru = lat.edotb() / cont, lat.edotb_unimproved() / cont
+        sign.append(np.sign(ri))
+        err_i.append(abs(abs(ri) - 1))
+        err_u.append(abs(abs(ru) - 1))
+    assert len(set(sign)) == 1
+    assert err_i[2] < 2e-4, err_i
+    assert 12 < err_i[1] / err_i[2] < 20, err_i        # O(a^4): 16
+    assert 3.5 < err_u[1] / err_u[2] < 4.5, err_u       # O(a^2): 4
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_edotb_map_sums_to_total(nc):
+    lat = O.Lattice(nc, 4)
+    lat.random(51, 0.6, 0.5, restore=False)
+    assert abs(lat.edotb_map().sum() - lat.edotb()) < 1e-12
+    assert abs(lat.ncsdot() - lat.edotb() / (8 * np.pi ** 2)) < 1e-15
+
+
+def test_langevin_equipartition():
+    nc, T = 2, 0.5
+    lat = O.Lattice(nc, 6)
+    lat.random(61, 0.3, 0.0, restore=False)
+    lat.langevin(300, 0.05, 1.0, T, seed=99, step0=0)
+    acc = []
+    for k in range(300):
+        lat.langevin(1, 0.05, 1.0, T, seed=99, step0=300 + k)
+        acc.append(lat.bulk()[:3].mean())     # <sum_a E^a E^a> per link
+    ratio = np.mean(acc) / (O.ne(nc) * T)
+    assert abs(ratio - 1) < 0.06, ratio
+
+
+def test_langevin_noise_variance_and_update_rule():
+    # one Langevin step with U = 1 (zero force): E' = (1 - damping dt) E + sqrt(dt) Xi
+    lat = O.Lattice(3, 3)
+    lat.random(71, 0.0, 0.4, restore=False)
+    E0 = lat.E.copy()
+    Xi = lat.noise(5, 3, np.sqrt(2 * 0.1 * 0.5))
+    assert abs(Xi.std() - np.sqrt(0.1)) < 0.02
+    lat.langevin_xi(Xi, 0.05, 0.1)
+    np.testing.assert_allclose(lat.E, (1 - 0.1 * 0.05) * E0 + np.sqrt(0.05) * Xi, atol=1e-15)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_gauss_restore_projects(nc):
+    lat = O.Lattice(nc, 4)
+    it, fm = lat.random(81, 0.5, 0.5)
+    assert 0 < it < 10000 and fm <= 1e-14
+    it2, _ = lat.gauss_restore()
+    assert it2 == 0
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_flow_descends_along_force(nc):
+    lat = O.Lattice(nc, 4)
+    lat.random(91, 0.6, 0.0, restore=False)
+    ref = lat.copy()
+    lf = lat.copy()
+    lf.leapfrog(1, 1e-3)
+    lat.flow_force()
+    np.testing.assert_allclose(lat.E, lf.E / 1e-3, atol=1e-12)
+    lat = ref.copy()
+    hb = [magnetic_energy(lat)]
+    for _ in range(10):
+        lat.flow(1, 0.02)
+        hb.append(magnetic_energy(lat))
+    assert all(b < a for a, b in zip(hb, hb[1:]))
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_geodesic_end_points(nc):
+    lat = O.Lattice(nc, 3)
+    lat.random(101, 0.5, 0.0, restore=False)
+    Uold = lat.U.copy()
+    lat.random(103, 0.5, 0.0, restore=False)
+    Unew = lat.U.copy()
+    U0, Em = lat.geodesic(Uold, Unew, 0.0)
+    U1, _ = lat.geodesic(Uold, Unew, 1.0)
+    np.testing.assert_allclose(U0, Uold, atol=1e-13)
+    np.testing.assert_allclose(U1, Unew, atol=1e-12)
+    # E_mid is the field whose unit-time link update maps Uold onto Unew
+    t = O.Lattice(nc, 3)
+    t.U[:], t.E[:] = Uold, Em
+    t.p.nthreads = 1
+    s, n = O.su(nc), O.ne(nc)
+    for l in range(3 * t.V):
+        got = O.mul(nc, O.expi(nc, -1.0, Em[n * l:n * (l + 1)]), Uold[s * l:s * (l + 1)])
+        np.testing.assert_allclose(got, Unew[s * l:s * (l + 1)], atol=1e-12)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_cooling_delta_ncs_is_simpson_on_the_geodesic(nc):
+    lat = O.Lattice(nc, 4)
+    lat.random(111, 0.5, 0.0, restore=False)
+    Uold = lat.U.copy()
+    rng = np.random.default_rng(7)
+    Ef = rng.normal(0, 0.15, lat.E.size)
+    s, n = O.su(nc), O.ne(nc)
+
+    def path(t):
+        out = np.empty_like(Uold)
+        for l in range(3 * lat.V):
+            out[s * l:s * (l + 1)] = O.mul(nc, O.expi(nc, -t, Ef[n * l:n * (l + 1)]), Uold[s * l:s * (l + 1)])
+        return out
+    Unew = path(1.0)
+    dn = lat.delta_ncs_cooling(Uold, Unew)
+    x, w = np.polynomial.legendre.leggauss(8)
+    exact = sum(0.5 * wi * lat.ncsdot(path(0.5 * (xi + 1)), Ef) for xi, wi in zip(x, w))
+    assert abs(dn - exact) < 2e-3 * abs(exact) + 1e-12, (dn, exact)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_random_initial_conditions(nc):
+    lat = O.Lattice(nc, 6)
+    lat.random(1001, 0.2, 0.3, restore=False)
+    assert lat.unitarity()[0] < 1e-14
+    n = O.ne(nc)
+    assert abs(lat.E.std() - 0.3) < 0.01
+    eps = np.concatenate([-O.logi(nc, -1.0, lat.U[O.su(nc) * l:O.su(nc) * (l + 1)]) for l in range(3 * lat.V)])
+    assert abs(eps.std() - 0.2) < 0.01
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_reunitarize(nc):
+    lat = O.Lattice(nc, 2)
+    lat.random(1201, 0.5, 0.0, restore=False)
+    W = lat.U.copy()
+    lat.U *= 1.0 + 1e-3 * np.random.default_rng(1).normal(size=lat.U.size)
+    assert lat.unitarity()[0] > 1e-5
+    lat.reunitarize()
+    assert lat.unitarity()[0] < 1e-14
+    assert np.abs(lat.U - W).max() < 1e-2
