@@ -1,0 +1,35 @@
+"""CPU-side checks of the drop-in boundary: the C-ABI library loads and exports
+every entry point that include/tcym.h declares (no compute calls here)."""
+import os
+import re
+
+import topologicalcymsolver_amd as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "tcym.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(tcym_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_what_the_wrapper_binds():
+    assert declared_symbols() == sorted(T.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = T.load_library()
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_version_string():
+    assert b"gfx950" in T.load_library().tcym_version()
+
+
+def test_library_is_gfx950_only():
+    # the code object bundled in the library targets gfx950 and nothing else
+    data = open(T.LIB_PATH, "rb").read()
+    targets = set(re.findall(rb"amdgcn-amd-amdhsa--(gfx[0-9a-f]+)", data))
+    assert targets == {b"gfx950"}, targets

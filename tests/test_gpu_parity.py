@@ -1,0 +1,240 @@
+"""GPU parity: the HIP library (through the C ABI) against the oracle on the
+same seeded inputs.
+
+FP64 form: per-kernel results agree with the oracle to <= 1e-12 absolute on
+O(1) quantities (the two sides use different but exact exp/log algorithms and
+a different summation order, so agreement is at rounding level, not bitwise);
+trajectories to relative 1e-10; Gauss and unitarity violations, which are
+rounding-level quantities, to absolute 1e-12.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+import topologicalcymsolver_amd as T
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(2, (6, 7, 8)), (3, (5, 6, 4))]
+
+
+def pair(nc, N, seed, su, se, restore=False, precision=T.FP64):
+    lat = O.Lattice(nc, N)
+    lat.random(seed, su, se, restore=restore)
+    gpu = T.Solver(nc, N, precision)
+    gpu.set_links(lat.U)
+    gpu.set_efield(lat.E)
+    return lat, gpu
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_upload_download_round_trip(nc, N):
+    lat, gpu = pair(nc, N, 3, 0.7, 0.4)
+    np.testing.assert_array_equal(gpu.get_links(), lat.U)
+    np.testing.assert_array_equal(gpu.get_efield(), lat.E)
+    gpu.copy_links(T.U_AUX, T.U_HOT)
+    np.testing.assert_array_equal(gpu.get_links(T.U_AUX), lat.U)
+    gpu.zero_efield(T.E_AUX)
+    assert not gpu.get_efield(T.E_AUX).any()
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_random_initial_conditions_match_oracle(nc, N):
+    lat = O.Lattice(nc, N)
+    lat.random(1234, 0.2, 0.3, restore=False)
+    gpu = T.Solver(nc, N)
+    gpu.random_links(1234, 0.2)
+    gpu.random_efield(1234, 0.3)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-14)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-14)
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_leapfrog_matches_oracle(nc, N):
+    lat, gpu = pair(nc, N, 7, 0.6, 0.5)
+    lat.leapfrog(10, 0.05)
+    gpu.dynamics_update(10, 0.05)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-12)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-12)
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_split_updates_match_oracle(nc, N):
+    lat, gpu = pair(nc, N, 8, 0.6, 0.5)
+    lat.leapfrog(1, 0.03)
+    gpu.update_electric_fields(0.03)
+    gpu.update_gauge_links(0.03)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-13)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-13)
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_langevin_matches_oracle(nc, N):
+    lat, gpu = pair(nc, N, 9, 0.5, 0.3)
+    lat.langevin(6, 0.05, 0.1, 0.5, seed=77, step0=4)
+    gpu.langevin_update(6, 0.05, 0.1, 0.5, seed=77, step0=4)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-12)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-12)
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_langevin_with_noise_field_matches_in_kernel_noise(nc, N):
+    lat, gpu = pair(nc, N, 10, 0.5, 0.3)
+    gpu.copy_links(T.U_AUX, T.U_HOT)
+    gpu.copy_efield(T.E_AUX, T.E_HOT)
+    gpu.langevin_update(1, 0.05, 1.0, 0.5, seed=5, step0=11)
+    gpu.random_gaussian_noise(5, 11, np.sqrt(2 * 1.0 * 0.5))
+    Xi = lat.noise(5, 11, np.sqrt(2 * 1.0 * 0.5))
+    np.testing.assert_allclose(gpu.get_efield(T.E_XI), Xi, atol=1e-14)
+    gpu.langevin_update_fields(0.05, 1.0, u=T.U_AUX, e=T.E_AUX)
+    np.testing.assert_allclose(gpu.get_links(T.U_AUX), gpu.get_links(), atol=1e-15)
+    np.testing.assert_allclose(gpu.get_efield(T.E_AUX), gpu.get_efield(), atol=1e-15)
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_flow_matches_oracle(nc, N):
+    lat, gpu = pair(nc, N, 11, 0.6, 0.0)
+    lat.flow(5, 0.02)
+    gpu.gradient_flow(5, 0.02, u=T.U_HOT, e=T.E_HOT)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-12)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-12)
+    # fused flow without storing the force gives the same links
+    lat2, gpu2 = pair(nc, N, 11, 0.6, 0.0)
+    gpu2.gradient_flow(5, 0.02, u=T.U_HOT, e=-1)
+    np.testing.assert_allclose(gpu2.get_links(), lat.U, atol=1e-12)
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_calibration_flow_matches_oracle(nc, N):
+    lat, gpu = pair(nc, N, 12, 0.6, 0.0)
+    d_or = lat.flow_calibration(4, 0.05)
+    d_gpu = gpu.gradient_flow(4, 0.05, calibrate=True, u=T.U_HOT, e=T.E_HOT)
+    assert abs(d_gpu - d_or) < 1e-12 * max(1.0, abs(d_or))
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-12)
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_observables_match_oracle(nc, N):
+    lat, gpu = pair(nc, N, 13, 0.7, 0.6)
+    for improved in (True, False):
+        want = lat.edotb() if improved else lat.edotb_unimproved()
+        got = gpu.edotb(improved)
+        assert abs(got - want) < 1e-12 * max(1.0, abs(want)), (improved, got, want)
+    np.testing.assert_allclose(gpu.edotb_map(), lat.edotb_map(), atol=1e-12)
+    assert abs(gpu.ncs_dot() - lat.ncsdot()) < 1e-13
+    np.testing.assert_allclose(gpu.bulk_update(), lat.bulk(), rtol=1e-13, atol=1e-14)
+    gm, gr = gpu.gauss_violation()
+    om, orr = lat.gauss()
+    assert abs(gm - om) < 1e-12 and abs(gr - orr) < 1e-12
+    um, ur = gpu.unitarity_violation()
+    vm, vr = lat.unitarity()
+    assert abs(um - vm) < 1e-12 and abs(ur - vr) < 1e-12
+    assert abs(gpu.vacuum_ncs() - lat.vacuum_ncs()) < 1e-13
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_gauss_restore_matches_oracle(nc, N):
+    lat, gpu = pair(nc, N, 14, 0.5, 0.5)
+    it_o, fm_o = lat.gauss_restore()
+    it_g, fm_g = gpu.gauss_law_restore()
+    assert it_g == it_o, (it_g, it_o)
+    assert fm_g <= 1e-14 and fm_o <= 1e-14
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-12)
+
+
+def test_gauss_restore_reports_iteration_cap():
+    lat, gpu = pair(2, (6, 6, 6), 15, 0.5, 0.5)
+    it, fm = gpu.gauss_law_restore(maxiter=2, check=False)
+    assert it == 2 and fm > 1e-14
+    with pytest.raises(T.TcymError) as ei:
+        gpu.gauss_law_restore(maxiter=1)
+    assert ei.value.code == T.E_NOCONV
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_geodesic_and_cooling_delta_ncs_match_oracle(nc, N):
+    lat, gpu = pair(nc, N, 16, 0.5, 0.0)
+    Uold = lat.U.copy()
+    lat2 = O.Lattice(nc, N)
+    lat2.random(17, 0.5, 0.0, restore=False)
+    Unew = lat2.U.copy()
+    gpu.set_links(Uold, T.U_OLD)
+    gpu.set_links(Unew, T.U_NEW)
+    gpu.geodesic_interpolation(0.3)
+    Um, Em = lat.geodesic(Uold, Unew, 0.3)
+    np.testing.assert_allclose(gpu.get_links(T.U_MID), Um, atol=1e-12)
+    np.testing.assert_allclose(gpu.get_efield(T.E_MID), Em, atol=1e-12)
+    d_or = lat.delta_ncs_cooling(Uold, Unew)
+    d_gpu = gpu.delta_ncs_cooling()
+    assert abs(d_gpu - d_or) < 1e-12 * max(1.0, abs(d_or))
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_cooling_method_update_matches_oracle(nc, N):
+    # ChernSimonsNumber::CoolingMethod: cool hot links 8 flow steps -> UNew,
+    # Delta N_CS between consecutive cooled configurations, UOld <- UNew
+    lat, gpu = pair(nc, N, 18, 0.5, 0.4)
+    gpu.cooling_method_start(8, 0.05)
+    cold = lat.copy()
+    cold.flow(8, 0.05)
+    Uold = cold.U.copy()
+    np.testing.assert_allclose(gpu.get_links(T.U_OLD), Uold, atol=1e-12)
+    lat.leapfrog(5, 0.05)
+    gpu.dynamics_update(5, 0.05)
+    d_gpu = gpu.cooling_method_update(8, 0.05, measure=True)
+    cold = lat.copy()
+    cold.flow(8, 0.05)
+    d_or = lat.delta_ncs_cooling(Uold, cold.U)
+    assert abs(d_gpu - d_or) < 1e-11 * max(1.0, abs(d_or)), (d_gpu, d_or)
+    np.testing.assert_allclose(gpu.get_links(T.U_OLD), cold.U, atol=1e-12)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_fp32_form_tracks_fp64(nc):
+    N = (8, 8, 8)
+    lat, g64 = pair(nc, N, 19, 0.3, 0.3, restore=True)
+    g32 = T.Solver(nc, N, T.FP32)
+    g32.set_links(lat.U)
+    g32.set_efield(lat.E)
+    g64.dynamics_update(40, 0.05)
+    g32.dynamics_update(40, 0.05)
+    assert g32.unitarity_violation()[0] < 2e-6          # FP64 reunitarisation after every step
+    assert abs(g32.energy() - g64.energy()) < 1e-5 * g64.energy()
+    np.testing.assert_allclose(g32.get_links(), g64.get_links(), atol=2e-4)
+    assert abs(g32.ncs_dot() - g64.ncs_dot()) < 1e-5
+    # FP64 Gauss restoration of the FP32 fields
+    g32.random_efield(5, 0.3)
+    it, fm = g32.gauss_law_restore()
+    assert fm <= 1e-14
+    assert g32.gauss_violation()[1] < 1e-6               # stored back in FP32
+
+
+def test_trajectory_su2_16_matches_oracle():
+    """SU(2) 16^3, sigma 0.2, sigma_E 0.3 Gauss-projected, 200 leapfrog steps
+    dt = 0.05 (the reference's CPU-runnable configuration): energy and the
+    hot-trajectory N_CS integral dN += dt/2 [NCsDot(U_n, E_n+1/2) + NCsDot(U_n+1, E_n+1/2)]
+    agree with the oracle to relative 1e-10; Gauss/unitarity to 1e-12 absolute."""
+    nc, N, dt = 2, (16, 16, 16), 0.05
+    lat = O.Lattice(nc, N)
+    lat.random(1001, 0.2, 0.3, restore=True)
+    gpu = T.Solver(nc, N)
+    gpu.set_links(lat.U)
+    gpu.set_efield(lat.E)
+    n_or = n_gpu = 0.0
+    e_or, e_gpu, ncs_or, ncs_gpu = [], [], [], []
+    for step in range(200):
+        U0 = lat.U.copy()
+        lat.leapfrog(1, dt)
+        n_or += 0.5 * dt * (lat.ncsdot(U0, lat.E) + lat.ncsdot())
+        gpu.copy_links(T.U_AUX, T.U_HOT)
+        gpu.dynamics_update(1, dt)
+        n_gpu += 0.5 * dt * (gpu.ncs_dot(u=T.U_AUX) + gpu.ncs_dot())
+        if step % 10 == 9:
+            e_or.append(lat.energy()); e_gpu.append(gpu.energy())
+            ncs_or.append(n_or); ncs_gpu.append(n_gpu)
+    e_or, e_gpu = np.array(e_or), np.array(e_gpu)
+    assert np.abs(e_gpu - e_or).max() <= 1e-10 * np.abs(e_or).max()
+    scale = max(np.abs(ncs_or).max(), 1e-12)
+    assert np.abs(np.array(ncs_gpu) - np.array(ncs_or)).max() <= 1e-10 * scale, (ncs_gpu[-1], ncs_or[-1])
+    assert abs(gpu.gauss_violation()[1] - lat.gauss()[1]) < 1e-12
+    assert abs(gpu.unitarity_violation()[0] - lat.unitarity()[0]) < 1e-12

@@ -1,0 +1,292 @@
+"""MI355X (gfx950) implementation of TopologicalCYMSolver's real-time lattice
+update loop, driven through the C ABI of libtcym.so (include/tcym.h).
+
+`Solver` owns one device context.  Its methods carry the reference's names
+(Dynamics::Update -> dynamics_update, ChernSimonsNumber::NCsDot -> ncs_dot,
+...) and take/return host arrays in the reference's AoS layout
+(U[s_U*(mu + 3*site)], E[a + n_E*(mu + 3*site)], site = x + Nx*(y + Ny*z)).
+There is no CPU fallback: without the built library or a gfx950 device every
+call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libtcym.so")
+
+OK, E_ARG, E_HIP, E_NOCONV, E_NONFINITE, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+FP64, FP32 = 0, 1
+U_HOT, U_COOL, U_OLD, U_NEW, U_MID, U_AUX = range(6)
+E_HOT, E_COOL, E_MID, E_XI, E_AUX = range(5)
+
+# every exported symbol of include/tcym.h
+EXPORTS = [
+    "tcym_create", "tcym_destroy", "tcym_last_error", "tcym_version",
+    "tcym_set_links", "tcym_get_links", "tcym_set_efield", "tcym_get_efield",
+    "tcym_copy_links", "tcym_copy_efield", "tcym_zero_efield", "tcym_unit_links",
+    "tcym_random_links", "tcym_random_efield",
+    "tcym_leapfrog", "tcym_update_efield", "tcym_update_links", "tcym_langevin", "tcym_langevin_xi",
+    "tcym_noise", "tcym_flow", "tcym_cool", "tcym_cooling_start", "tcym_cooling_update", "tcym_reunitarize",
+    "tcym_edotb", "tcym_ncsdot", "tcym_edotb_map", "tcym_bulk", "tcym_gauss", "tcym_unitarity",
+    "tcym_gauss_restore", "tcym_geodesic", "tcym_delta_ncs_cooling", "tcym_vacuum_ncs",
+    "tcym_timing", "tcym_kernel_time", "tcym_timing_reset", "tcym_sync", "tcym_device_bytes",
+]
+
+
+class TcymError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("tcym error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("nc", ctypes.c_int), ("precision", ctypes.c_int), ("N", ctypes.c_int * 3),
+                ("a", ctypes.c_double * 3), ("aScale", ctypes.c_double), ("device", ctypes.c_int)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libtcym.so and declare its signatures.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libtcym.so is not built (%s); run python -m topologicalcymsolver_amd.build" % path)
+    L = ctypes.CDLL(path)
+    V, I, D, P = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(Params)
+    U32, U64, LL = ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_longlong)
+    sig = {
+        "tcym_create": [P, ctypes.POINTER(V)], "tcym_destroy": [V],
+        "tcym_set_links": [V, I, _dp], "tcym_get_links": [V, I, _dp],
+        "tcym_set_efield": [V, I, _dp], "tcym_get_efield": [V, I, _dp],
+        "tcym_copy_links": [V, I, I], "tcym_copy_efield": [V, I, I], "tcym_zero_efield": [V, I],
+        "tcym_unit_links": [V, I], "tcym_random_links": [V, I, U64, D], "tcym_random_efield": [V, I, U64, D],
+        "tcym_leapfrog": [V, I, I, I, D], "tcym_update_efield": [V, I, I, D], "tcym_update_links": [V, I, I, D],
+        "tcym_langevin": [V, I, I, I, D, D, D, U64, U32], "tcym_langevin_xi": [V, I, I, I, D, D],
+        "tcym_noise": [V, I, U64, U32, D], "tcym_flow": [V, I, I, I, D, I, _dp],
+        "tcym_cool": [V, I, I, I, D], "tcym_cooling_start": [V, I, I, D],
+        "tcym_cooling_update": [V, I, I, D, I, _dp], "tcym_reunitarize": [V, I],
+        "tcym_edotb": [V, I, I, I, _dp], "tcym_ncsdot": [V, I, I, _dp], "tcym_edotb_map": [V, I, I, _dp],
+        "tcym_bulk": [V, I, I, _dp], "tcym_gauss": [V, I, I, _dp, _dp], "tcym_unitarity": [V, I, _dp, _dp],
+        "tcym_gauss_restore": [V, I, I, I, D, ctypes.POINTER(I), _dp],
+        "tcym_geodesic": [V, I, I, I, I, D], "tcym_delta_ncs_cooling": [V, I, I, _dp],
+        "tcym_vacuum_ncs": [V, I, _dp], "tcym_timing": [V, I],
+        "tcym_kernel_time": [V, ctypes.c_char_p, _dp, LL], "tcym_timing_reset": [V], "tcym_sync": [V],
+        "tcym_device_bytes": [V, LL],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = I
+    L.tcym_last_error.argtypes = [V]
+    L.tcym_last_error.restype = ctypes.c_char_p
+    L.tcym_version.argtypes = []
+    L.tcym_version.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _arr(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError("expected %d values, got %d" % (n, a.size))
+    return a
+
+
+class Solver:
+    """One lattice context on one GPU (Lattice::Init + the GLinks/EFields singletons)."""
+
+    def __init__(self, nc, N, precision=FP64, a=(1.0, 1.0, 1.0), aScale=1.0, device=0):
+        self.L = load_library()
+        if isinstance(N, int):
+            N = (N, N, N)
+        self.nc, self.N, self.precision = nc, tuple(N), precision
+        self.V = N[0] * N[1] * N[2]
+        self.su = 4 if nc == 2 else 18
+        self.ne = 3 if nc == 2 else 8
+        p = Params(nc, precision, (ctypes.c_int * 3)(*N), (ctypes.c_double * 3)(*a), aScale, device)
+        h = ctypes.c_void_p()
+        r = self.L.tcym_create(ctypes.byref(p), ctypes.byref(h))
+        if r != OK:
+            raise TcymError(r, "tcym_create failed (group %d, N=%s, device %d)" % (nc, N, device))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tcym_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, r):
+        if r != OK:
+            raise TcymError(r, self.L.tcym_last_error(self.h).decode())
+        return r
+
+    # ---- fields ----
+    def set_links(self, U, slot=U_HOT):
+        U = _arr(U, 3 * self.V * self.su)
+        self._ck(self.L.tcym_set_links(self.h, slot, U.ctypes.data_as(_dp)))
+
+    def get_links(self, slot=U_HOT):
+        U = np.empty(3 * self.V * self.su)
+        self._ck(self.L.tcym_get_links(self.h, slot, U.ctypes.data_as(_dp)))
+        return U
+
+    def set_efield(self, E, slot=E_HOT):
+        E = _arr(E, 3 * self.V * self.ne)
+        self._ck(self.L.tcym_set_efield(self.h, slot, E.ctypes.data_as(_dp)))
+
+    def get_efield(self, slot=E_HOT):
+        E = np.empty(3 * self.V * self.ne)
+        self._ck(self.L.tcym_get_efield(self.h, slot, E.ctypes.data_as(_dp)))
+        return E
+
+    def copy_links(self, dst, src):
+        self._ck(self.L.tcym_copy_links(self.h, dst, src))
+
+    def copy_efield(self, dst, src):
+        self._ck(self.L.tcym_copy_efield(self.h, dst, src))
+
+    def zero_efield(self, slot=E_HOT):
+        self._ck(self.L.tcym_zero_efield(self.h, slot))
+
+    def unit_links(self, slot=U_HOT):
+        self._ck(self.L.tcym_unit_links(self.h, slot))
+
+    def random_links(self, seed, sigma, slot=U_HOT):
+        self._ck(self.L.tcym_random_links(self.h, slot, seed, sigma))
+
+    def random_efield(self, seed, sigma, slot=E_HOT):
+        self._ck(self.L.tcym_random_efield(self.h, slot, seed, sigma))
+
+    # ---- updates (reference names) ----
+    def dynamics_update(self, nsteps=1, dt=0.025, u=U_HOT, e=E_HOT):
+        """Dynamics::Update (DYNAMICS/Dynamics.cpp:204-218), dTau = dt."""
+        self._ck(self.L.tcym_leapfrog(self.h, u, e, nsteps, dt))
+
+    def update_electric_fields(self, dt=0.025, u=U_HOT, e=E_HOT):
+        self._ck(self.L.tcym_update_efield(self.h, u, e, dt))
+
+    def update_gauge_links(self, dt=0.025, u=U_HOT, e=E_HOT):
+        self._ck(self.L.tcym_update_links(self.h, u, e, dt))
+
+    def langevin_update(self, nsteps=1, dt=0.001, damping=1.0, T=0.5, seed=0, step0=0, u=U_HOT, e=E_HOT):
+        """LangevinDynamics::Update (DYNAMICS/LangevinDynamics.cpp:320-347)."""
+        self._ck(self.L.tcym_langevin(self.h, u, e, nsteps, dt, damping, T, seed, step0))
+
+    def langevin_update_fields(self, dt=0.001, damping=1.0, xi=E_XI, u=U_HOT, e=E_HOT):
+        """LangevinDynamics::UpdateFields with LangevinVariables::Xi (DYNAMICS/LangevinDynamics.cpp:300-311)."""
+        self._ck(self.L.tcym_langevin_xi(self.h, u, e, xi, dt, damping))
+
+    def random_gaussian_noise(self, seed, step, stddev, xi=E_XI):
+        self._ck(self.L.tcym_noise(self.h, xi, seed, step, stddev))
+
+    def gradient_flow(self, nsteps=1, dtau=0.125, calibrate=False, u=U_COOL, e=E_COOL):
+        """GradientFlow::StandardUpdate / CalibrationUpdate (GRADIENTFLOW/GradientFlow.cpp:150-188)."""
+        d = ctypes.c_double(0.0)
+        self._ck(self.L.tcym_flow(self.h, u, e, nsteps, dtau, 1 if calibrate else 0, ctypes.byref(d)))
+        return d.value
+
+    def cool_n_save(self, nsteps, dtau, u_hot=U_HOT, u_cold=U_NEW):
+        """Cooling::CoolNSave (GRADIENTFLOW/Cooling.cpp:156-179)."""
+        self._ck(self.L.tcym_cool(self.h, u_hot, u_cold, nsteps, dtau))
+
+    def cooling_method_start(self, nsteps, dtau, u_hot=U_HOT):
+        self._ck(self.L.tcym_cooling_start(self.h, u_hot, nsteps, dtau))
+
+    def cooling_method_update(self, nsteps, dtau, measure=True, u_hot=U_HOT):
+        """ChernSimonsNumber::CoolingMethod::Update (TOPOLOGY/CoolingMethod.cpp:143-165); returns GetDeltaNCs(1)."""
+        d = ctypes.c_double(0.0)
+        self._ck(self.L.tcym_cooling_update(self.h, u_hot, nsteps, dtau, 1 if measure else 0, ctypes.byref(d)))
+        return d.value
+
+    def reunitarize(self, u=U_HOT):
+        self._ck(self.L.tcym_reunitarize(self.h, u))
+
+    # ---- observables ----
+    def edotb(self, improved=True, u=U_HOT, e=E_HOT):
+        d = ctypes.c_double()
+        self._ck(self.L.tcym_edotb(self.h, u, e, 1 if improved else 0, ctypes.byref(d)))
+        return d.value
+
+    def ncs_dot(self, u=U_HOT, e=E_HOT):
+        """ChernSimonsNumber::NCsDot (TOPOLOGY/ChernSimonsDerivative.cpp:31-49)."""
+        d = ctypes.c_double()
+        self._ck(self.L.tcym_ncsdot(self.h, u, e, ctypes.byref(d)))
+        return d.value
+
+    def edotb_map(self, u=U_HOT, e=E_HOT):
+        m = np.empty(self.V)
+        self._ck(self.L.tcym_edotb_map(self.h, u, e, m.ctypes.data_as(_dp)))
+        return m
+
+    def bulk_update(self, u=U_HOT, e=E_HOT):
+        """Observables::Bulk::Update: {ESqr[3], BSqr[3]} (OBSERVABLES/BulkObservables.cpp:133-143)."""
+        o = np.empty(6)
+        self._ck(self.L.tcym_bulk(self.h, u, e, o.ctypes.data_as(_dp)))
+        return o
+
+    def energy(self, u=U_HOT, e=E_HOT):
+        """Bulk::T00 for a = aScale = 1 (OBSERVABLES/BulkObservables.cpp:29-31)."""
+        return 0.5 * float(self.bulk_update(u, e).sum())
+
+    def gauss_violation(self, u=U_HOT, e=E_HOT):
+        m, r = ctypes.c_double(), ctypes.c_double()
+        self._ck(self.L.tcym_gauss(self.h, u, e, ctypes.byref(m), ctypes.byref(r)))
+        return m.value, r.value
+
+    def unitarity_violation(self, u=U_HOT):
+        m, r = ctypes.c_double(), ctypes.c_double()
+        self._ck(self.L.tcym_unitarity(self.h, u, ctypes.byref(m), ctypes.byref(r)))
+        return m.value, r.value
+
+    def gauss_law_restore(self, maxiter=10000, tol=1e-14, u=U_HOT, e=E_HOT, check=True):
+        """GaussLawRestoration::Restore (GAUSSLAWFIXING/GaussRestore.cpp:131-167)."""
+        it, fm = ctypes.c_int(), ctypes.c_double()
+        r = self.L.tcym_gauss_restore(self.h, u, e, maxiter, tol, ctypes.byref(it), ctypes.byref(fm))
+        if r != OK and (check or r != E_NOCONV):
+            self._ck(r)
+        return it.value, fm.value
+
+    def geodesic_interpolation(self, c, uold=U_OLD, unew=U_NEW, umid=U_MID, emid=E_MID):
+        self._ck(self.L.tcym_geodesic(self.h, uold, unew, umid, emid, c))
+
+    def delta_ncs_cooling(self, uold=U_OLD, unew=U_NEW):
+        d = ctypes.c_double()
+        self._ck(self.L.tcym_delta_ncs_cooling(self.h, uold, unew, ctypes.byref(d)))
+        return d.value
+
+    def vacuum_ncs(self, u=U_HOT):
+        d = ctypes.c_double()
+        self._ck(self.L.tcym_vacuum_ncs(self.h, u, ctypes.byref(d)))
+        return d.value
+
+    # ---- measurement ----
+    def timing(self, enable=True):
+        self._ck(self.L.tcym_timing(self.h, 1 if enable else 0))
+
+    def timing_reset(self):
+        self._ck(self.L.tcym_timing_reset(self.h))
+
+    def kernel_time(self, name):
+        t, n = ctypes.c_double(), ctypes.c_longlong()
+        self._ck(self.L.tcym_kernel_time(self.h, name.encode(), ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    def sync(self):
+        self._ck(self.L.tcym_sync(self.h))
+
+    def device_bytes(self):
+        b = ctypes.c_longlong()
+        self._ck(self.L.tcym_device_bytes(self.h, ctypes.byref(b)))
+        return b.value

@@ -1,0 +1,597 @@
+// C ABI (include/tcym.h): device context, field slots, orchestration of the
+// update/measurement kernels, deterministic reductions and kernel timing.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "tcym.h"
+#include "tcym_backend.hpp"
+
+using tcym::Backend;
+using tcym::Geom;
+using tcym::kBlock;
+
+namespace {
+
+// final reduction: block q reduces quantity q of nblk partials in a fixed order
+// (per-thread strided Neumaier sums, then a fixed shuffle tree)
+__global__ __launch_bounds__(256) void k_final(const double* __restrict__ part, long nblk, int nq, unsigned is_max,
+                                               double* __restrict__ out) {
+    const int q = blockIdx.x;
+    const bool mx = (is_max >> q) & 1;
+    double s = mx ? 0.0 : 0.0, c = 0.0;
+    for (long b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const double v = part[b*nq + q];
+        if (mx) s = fmax(s, v);
+        else {
+            const double t = s + v;
+            c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+            s = t;
+        }
+    }
+    if (!mx) s += c;
+    __shared__ double sm[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_down(s, o, 64);
+        s = mx ? fmax(s, w) : s + w;
+    }
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sm[0];
+        for (int w = 1; w < 4; w++) r = mx ? fmax(r, sm[w]) : r + sm[w];
+        out[q] = r;
+    }
+}
+
+struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+namespace tcym {
+void final_reduce(const Launcher& L, const double* part, long nblk, int nq, const int* is_max, double* out) {
+    unsigned m = 0;
+    for (int q = 0; q < nq; q++) if (is_max[q]) m |= 1u << q;
+    L("reduce", [&](hipStream_t st) { hipLaunchKernelGGL(k_final, dim3(nq), dim3(256), 0, st, part, nblk, nq, m, out); });
+}
+}  // namespace tcym
+
+struct tcym_ctx {
+    Geom g{};
+    int nc = 0, prec = 0, device = 0;
+    Backend* be = nullptr;     // storage precision
+    Backend* be64 = nullptr;   // FP64 (Gauss-law restoration of the FP32 form)
+    hipStream_t stream = nullptr;
+    void* U[TCYM_NUM_U] = {};
+    void* E[TCYM_NUM_E] = {};
+    void* Uscr = nullptr;
+    double* part = nullptr;
+    double* red = nullptr;       // device results
+    double* red_host = nullptr;  // pinned
+    double* stage = nullptr;     // AoS staging, 3 V max(ns, ne) doubles
+    double* G64 = nullptr;
+    double* U64 = nullptr;
+    double* E64 = nullptr;
+    double* map = nullptr;
+    long long bytes = 0;
+    std::string err;
+    bool timing = false;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> free_events;
+    std::map<std::string, std::pair<double, long long>> stats;
+    hipError_t last = hipSuccess;
+
+    long link_elems() const { return 3*g.V*be->ns; }
+    long field_elems() const { return 3*g.V*be->ne; }
+
+    hipEvent_t ev() {
+        if (!free_events.empty()) { hipEvent_t e = free_events.back(); free_events.pop_back(); return e; }
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void launch(const char* name, const tcym::Launch& f) {
+        if (last != hipSuccess) return;
+        if (timing) {
+            hipEvent_t a = ev(), b = ev();
+            hipEventRecord(a, stream);
+            f(stream);
+            hipEventRecord(b, stream);
+            pending.push_back({name, a, b});
+        } else {
+            f(stream);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) { last = e; err = std::string("kernel ") + name + ": " + hipGetErrorString(e); }
+    }
+    tcym::Launcher L() { return [this](const char* n, const tcym::Launch& f) { launch(n, f); }; }
+
+    int fail(int code, const std::string& m) { err = m; return code; }
+    int hip(hipError_t e, const char* what) {
+        if (e == hipSuccess) return TCYM_OK;
+        last = e;
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return TCYM_E_HIP;
+    }
+    int status() { return last == hipSuccess ? TCYM_OK : TCYM_E_HIP; }
+    int sync() {
+        if (last != hipSuccess) return TCYM_E_HIP;
+        return hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+    int alloc(void** p, size_t n, const char* what) {
+        if (*p) return TCYM_OK;
+        int r = hip(hipMalloc(p, n), what);
+        if (r == TCYM_OK) { bytes += (long long)n; r = hip(hipMemsetAsync(*p, 0, n, stream), what); }
+        return r;
+    }
+    int useU(int slot) {
+        if (slot < 0 || slot >= TCYM_NUM_U) return fail(TCYM_E_ARG, "link slot out of range");
+        return alloc(&U[slot], (size_t)link_elems()*be->elsize, "hipMalloc(links)");
+    }
+    int useE(int slot) {
+        if (slot < 0 || slot >= TCYM_NUM_E) return fail(TCYM_E_ARG, "field slot out of range");
+        return alloc(&E[slot], (size_t)field_elems()*be->elsize, "hipMalloc(efield)");
+    }
+    int scratch() { return alloc(&Uscr, (size_t)link_elems()*be->elsize, "hipMalloc(link scratch)"); }
+
+    // reduction: run partial kernel, final-reduce nq quantities, copy to host
+    template <class F>
+    int reduce(int nq, const int* is_max, F partial, double* out) {
+        long nblk = partial();
+        tcym::final_reduce(L(), part, nblk, nq, is_max, red);
+        if (last != hipSuccess) return TCYM_E_HIP;
+        int r = hip(hipMemcpyAsync(red_host, red, sizeof(double)*nq, hipMemcpyDeviceToHost, stream), "hipMemcpyAsync(result)");
+        if (r) return r;
+        if ((r = sync())) return r;
+        for (int q = 0; q < nq; q++) {
+            out[q] = red_host[q];
+            if (!isfinite(out[q])) { err = "non-finite reduction result"; return TCYM_E_NONFINITE; }
+        }
+        return TCYM_OK;
+    }
+    int edotb(int u, int e, int improved, double* out, bool map_out = false) {
+        int r;
+        if ((r = useU(u)) || (r = useE(e))) return r;
+        if (map_out && (r = alloc((void**)&map, sizeof(double)*(size_t)g.V, "hipMalloc(map)"))) return r;
+        const int mx[1] = {0};
+        return reduce(1, mx, [&] { return be->edotb(L(), g, improved, U[u], E[e], part, map_out ? map : nullptr); }, out);
+    }
+    double flow_dctau(double dtau) const {   // GRADIENTFLOW/GradientFlow.cpp:152
+        const double ac = g.a[0]*g.a[1]*g.a[2];
+        return dtau*pow(ac, 2.0/3.0)/(g.aScale*g.aScale);
+    }
+    void flow_cu(double dtau, double* cU) const {  // GradientFlow.cpp:109-113
+        const double ac = g.a[0]*g.a[1]*g.a[2], D = dtau*pow(ac, 2.0/3.0);
+        for (int mu = 0; mu < 3; mu++) cU[mu] = -(g.a[mu]*g.a[mu]*D)/(ac*g.aScale);
+    }
+    int ncsdot(int u, int e, double* out) {
+        double v;
+        int r = edotb(u, e, 1, &v);
+        if (r) return r;
+        *out = 0.125*v/(M_PI*M_PI);
+        return TCYM_OK;
+    }
+};
+
+extern "C" {
+
+const char* tcym_version(void) { return "tcym-mi355x 0.1 (gfx950)"; }
+
+const char* tcym_last_error(const tcym_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int tcym_create(const tcym_params* p, tcym_ctx** out) {
+    if (!p || !out) return TCYM_E_ARG;
+    *out = nullptr;
+    if ((p->nc != 2 && p->nc != 3) || (p->precision != TCYM_FP64 && p->precision != TCYM_FP32)) return TCYM_E_ARG;
+    for (int k = 0; k < 3; k++) if (p->N[k] < 3 || !(p->a[k] > 0)) return TCYM_E_ARG;
+    if (!(p->aScale > 0)) return TCYM_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= p->device || p->device < 0) return TCYM_E_HIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) return TCYM_E_HIP;
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return TCYM_E_UNSUPPORTED;
+    if (hipSetDevice(p->device) != hipSuccess) return TCYM_E_HIP;
+    tcym_ctx* c = new tcym_ctx();
+    c->nc = p->nc; c->prec = p->precision; c->device = p->device;
+    for (int k = 0; k < 3; k++) { c->g.n[k] = p->N[k]; c->g.a[k] = p->a[k]; }
+    c->g.aScale = p->aScale;
+    c->g.V = (long)p->N[0]*p->N[1]*p->N[2];
+    const int f32 = p->precision == TCYM_FP32;
+    c->be = p->nc == 2 ? tcym::make_backend_su2(f32) : tcym::make_backend_su3(f32);
+    c->be64 = f32 ? (p->nc == 2 ? tcym::make_backend_su2(0) : tcym::make_backend_su3(0)) : c->be;
+    int r = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    const long nblk = (c->g.V + kBlock - 1)/kBlock;
+    if (!r) r = c->alloc((void**)&c->part, sizeof(double)*(size_t)nblk*8, "hipMalloc(partials)");
+    if (!r) r = c->alloc((void**)&c->red, sizeof(double)*16, "hipMalloc(results)");
+    if (!r) r = c->hip(hipHostMalloc((void**)&c->red_host, sizeof(double)*16, hipHostMallocDefault), "hipHostMalloc");
+    if (!r) r = c->alloc((void**)&c->stage, sizeof(double)*(size_t)(3*c->g.V*std::max(c->be->ns, c->be->ne)), "hipMalloc(stage)");
+    if (!r) r = c->sync();
+    if (r) { tcym_destroy(c); return r; }
+    *out = c;
+    return TCYM_OK;
+}
+
+int tcym_destroy(tcym_ctx* c) {
+    if (!c) return TCYM_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& q : c->pending) { hipEventDestroy(q.a); hipEventDestroy(q.b); }
+    for (auto e : c->free_events) hipEventDestroy(e);
+    for (void* p : c->U) if (p) hipFree(p);
+    for (void* p : c->E) if (p) hipFree(p);
+    void* bufs[] = {c->Uscr, c->part, c->red, c->stage, c->G64, c->U64, c->E64, c->map};
+    for (void* p : bufs) if (p) hipFree(p);
+    if (c->red_host) hipHostFree(c->red_host);
+    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->be64 != c->be) delete c->be64;
+    delete c->be;
+    delete c;
+    return TCYM_OK;
+}
+
+int tcym_sync(tcym_ctx* c) { return c ? c->sync() : TCYM_E_ARG; }
+
+int tcym_device_bytes(tcym_ctx* c, long long* b) {
+    if (!c || !b) return TCYM_E_ARG;
+    *b = c->bytes;
+    return TCYM_OK;
+}
+
+// ---------------------------------------------------------------- fields
+int tcym_set_links(tcym_ctx* c, int slot, const double* U) {
+    if (!c || !U) return TCYM_E_ARG;
+    int r = c->useU(slot);
+    if (r) return r;
+    if ((r = c->hip(hipMemcpyAsync(c->stage, U, sizeof(double)*(size_t)c->link_elems(), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(links)"))) return r;
+    c->be->pack_links(c->L(), c->g, c->stage, c->U[slot]);
+    return c->sync();
+}
+int tcym_get_links(tcym_ctx* c, int slot, double* U) {
+    if (!c || !U) return TCYM_E_ARG;
+    int r = c->useU(slot);
+    if (r) return r;
+    c->be->unpack_links(c->L(), c->g, c->U[slot], c->stage);
+    if ((r = c->hip(hipMemcpyAsync(U, c->stage, sizeof(double)*(size_t)c->link_elems(), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(links)"))) return r;
+    return c->sync();
+}
+int tcym_set_efield(tcym_ctx* c, int slot, const double* E) {
+    if (!c || !E) return TCYM_E_ARG;
+    int r = c->useE(slot);
+    if (r) return r;
+    if ((r = c->hip(hipMemcpyAsync(c->stage, E, sizeof(double)*(size_t)c->field_elems(), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(efield)"))) return r;
+    c->be->pack_efield(c->L(), c->g, c->stage, c->E[slot]);
+    return c->sync();
+}
+int tcym_get_efield(tcym_ctx* c, int slot, double* E) {
+    if (!c || !E) return TCYM_E_ARG;
+    int r = c->useE(slot);
+    if (r) return r;
+    c->be->unpack_efield(c->L(), c->g, c->E[slot], c->stage);
+    if ((r = c->hip(hipMemcpyAsync(E, c->stage, sizeof(double)*(size_t)c->field_elems(), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(efield)"))) return r;
+    return c->sync();
+}
+int tcym_copy_links(tcym_ctx* c, int dst, int src) {
+    if (!c) return TCYM_E_ARG;
+    int r;
+    if ((r = c->useU(dst)) || (r = c->useU(src))) return r;
+    if (dst == src) return TCYM_OK;
+    c->be->copy(c->L(), c->link_elems(), c->U[src], c->U[dst]);
+    return c->sync();
+}
+int tcym_copy_efield(tcym_ctx* c, int dst, int src) {
+    if (!c) return TCYM_E_ARG;
+    int r;
+    if ((r = c->useE(dst)) || (r = c->useE(src))) return r;
+    if (dst == src) return TCYM_OK;
+    c->be->copy(c->L(), c->field_elems(), c->E[src], c->E[dst]);
+    return c->sync();
+}
+int tcym_zero_efield(tcym_ctx* c, int slot) {
+    if (!c) return TCYM_E_ARG;
+    int r = c->useE(slot);
+    if (r) return r;
+    if ((r = c->hip(hipMemsetAsync(c->E[slot], 0, (size_t)c->field_elems()*c->be->elsize, c->stream), "zero efield"))) return r;
+    return c->sync();
+}
+int tcym_unit_links(tcym_ctx* c, int slot) {
+    if (!c) return TCYM_E_ARG;
+    int r = c->useU(slot);
+    if (r) return r;
+    c->be->random_links(c->L(), c->g, 0, 0.0, c->U[slot]);   // exp(0) = 1
+    return c->sync();
+}
+int tcym_random_links(tcym_ctx* c, int slot, uint64_t seed, double sigma) {
+    if (!c) return TCYM_E_ARG;
+    int r = c->useU(slot);
+    if (r) return r;
+    c->be->random_links(c->L(), c->g, seed, sigma, c->U[slot]);
+    return c->sync();
+}
+int tcym_random_efield(tcym_ctx* c, int slot, uint64_t seed, double sigma) {
+    if (!c) return TCYM_E_ARG;
+    int r = c->useE(slot);
+    if (r) return r;
+    c->be->random_efield(c->L(), c->g, seed, 1u, 0u, sigma, c->E[slot]);
+    return c->sync();
+}
+int tcym_noise(tcym_ctx* c, int xi, uint64_t seed, uint32_t step, double stddev) {
+    if (!c) return TCYM_E_ARG;
+    int r = c->useE(xi);
+    if (r) return r;
+    c->be->random_efield(c->L(), c->g, seed, 2u, step, stddev, c->E[xi]);
+    return c->sync();
+}
+
+// ---------------------------------------------------------------- updates
+static int run_update(tcym_ctx* c, int mode, int u, int e, int xi, int nsteps, double dt, double damping,
+                      double noise_sd, uint64_t seed, uint32_t step0, bool write_e) {
+    int r;
+    if ((r = c->useU(u)) || (r = c->scratch())) return r;
+    if (e >= 0 && (r = c->useE(e))) return r;
+    if (mode != 3 && e < 0) return c->fail(TCYM_E_ARG, "E slot required");
+    if (mode == 2 && (r = c->useE(xi))) return r;
+    if (nsteps < 0) return c->fail(TCYM_E_ARG, "nsteps < 0");
+    void* E = e >= 0 ? c->E[e] : nullptr;
+    const void* X = mode == 2 ? c->E[xi] : nullptr;
+    for (int k = 0; k < nsteps; k++) {
+        c->be->update(c->L(), c->g, mode, true, write_e, dt, damping, noise_sd, seed, step0 + (uint32_t)k,
+                      c->U[u], c->Uscr, E, X);
+        std::swap(c->U[u], c->Uscr);
+        if (c->last != hipSuccess) return TCYM_E_HIP;
+    }
+    return c->sync();
+}
+
+int tcym_leapfrog(tcym_ctx* c, int u, int e, int nsteps, double dt) {
+    if (!c) return TCYM_E_ARG;
+    return run_update(c, 0, u, e, -1, nsteps, dt, 0.0, 0.0, 0, 0, true);
+}
+int tcym_update_efield(tcym_ctx* c, int u, int e, double dt) {
+    if (!c) return TCYM_E_ARG;
+    int r;
+    if ((r = c->useU(u)) || (r = c->useE(e))) return r;
+    c->be->update(c->L(), c->g, 0, false, true, dt, 0.0, 0.0, 0, 0, c->U[u], nullptr, c->E[e], nullptr);
+    return c->sync();
+}
+int tcym_update_links(tcym_ctx* c, int u, int e, double dt) {
+    if (!c) return TCYM_E_ARG;
+    int r;
+    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->scratch())) return r;
+    const double ac = c->g.a[0]*c->g.a[1]*c->g.a[2];
+    double cU[3];
+    for (int mu = 0; mu < 3; mu++) cU[mu] = -dt*c->g.a[mu]*c->g.a[mu]/ac;   // DYNAMICS/Dynamics.cpp:102-104
+    c->be->link_update(c->L(), c->g, cU, c->U[u], c->Uscr, c->E[e]);
+    std::swap(c->U[u], c->Uscr);
+    return c->sync();
+}
+int tcym_langevin(tcym_ctx* c, int u, int e, int nsteps, double dt, double damping, double T,
+                  uint64_t seed, uint32_t step0) {
+    if (!c) return TCYM_E_ARG;
+    if (!(T >= 0) || !(damping >= 0)) return c->fail(TCYM_E_ARG, "negative temperature or damping");
+    return run_update(c, 1, u, e, -1, nsteps, dt, damping, sqrt(2.0*damping*T), seed, step0, true);
+}
+int tcym_langevin_xi(tcym_ctx* c, int u, int e, int xi, double dt, double damping) {
+    if (!c) return TCYM_E_ARG;
+    return run_update(c, 2, u, e, xi, 1, dt, damping, 0.0, 0, 0, true);
+}
+int tcym_flow(tcym_ctx* c, int u, int e, int nsteps, double dtau, int calibrate, double* dncs) {
+    if (!c) return TCYM_E_ARG;
+    if (!calibrate) return run_update(c, 3, u, e, -1, nsteps, dtau, 0.0, 0.0, 0, 0, e >= 0);
+    int r;
+    if (e < 0) return c->fail(TCYM_E_ARG, "calibration flow needs an E slot");
+    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->scratch())) return r;
+    const double D = c->flow_dctau(dtau);
+    double cU[3], acc = 0.0, v;
+    c->flow_cu(dtau, cU);
+    for (int k = 0; k < nsteps; k++) {
+        // GradientFlow::CalibrationUpdate (GRADIENTFLOW/GradientFlow.cpp:168-188)
+        c->be->update(c->L(), c->g, 3, false, true, dtau, 0.0, 0.0, 0, 0, c->U[u], nullptr, c->E[e], nullptr);
+        if ((r = c->ncsdot(u, e, &v))) return r;
+        acc += 0.5*D*v;
+        c->be->link_update(c->L(), c->g, cU, c->U[u], c->Uscr, c->E[e]);
+        std::swap(c->U[u], c->Uscr);
+        if ((r = c->ncsdot(u, e, &v))) return r;
+        acc += 0.5*D*v;
+    }
+    if (dncs) *dncs += acc;
+    return c->sync();
+}
+int tcym_cool(tcym_ctx* c, int u_hot, int u_cold, int nsteps, double dtau) {
+    if (!c) return TCYM_E_ARG;
+    int r;
+    if ((r = tcym_copy_links(c, u_cold, u_hot))) return r;
+    // the flowed E field of Cooling::DynamicFields is discarded by CoolNSave, so
+    // the fused flow keeps the force in registers (GRADIENTFLOW/Cooling.cpp:156-179)
+    return run_update(c, 3, u_cold, -1, -1, nsteps, dtau, 0.0, 0.0, 0, 0, false);
+}
+int tcym_reunitarize(tcym_ctx* c, int u) {
+    if (!c) return TCYM_E_ARG;
+    int r = c->useU(u);
+    if (r) return r;
+    c->be->reunitarize(c->L(), c->g, c->U[u]);
+    return c->sync();
+}
+
+// ---------------------------------------------------------------- observables
+int tcym_edotb(tcym_ctx* c, int u, int e, int improved, double* out) {
+    if (!c || !out) return TCYM_E_ARG;
+    return c->edotb(u, e, improved ? 1 : 0, out);
+}
+int tcym_ncsdot(tcym_ctx* c, int u, int e, double* out) {
+    if (!c || !out) return TCYM_E_ARG;
+    return c->ncsdot(u, e, out);
+}
+int tcym_edotb_map(tcym_ctx* c, int u, int e, double* map) {
+    if (!c || !map) return TCYM_E_ARG;
+    double tot;
+    int r = c->edotb(u, e, 1, &tot, true);
+    if (r) return r;
+    if ((r = c->hip(hipMemcpyAsync(map, c->map, sizeof(double)*(size_t)c->g.V, hipMemcpyDeviceToHost, c->stream), "copy map"))) return r;
+    return c->sync();
+}
+int tcym_bulk(tcym_ctx* c, int u, int e, double out[6]) {
+    if (!c || !out) return TCYM_E_ARG;
+    int r;
+    if ((r = c->useU(u)) || (r = c->useE(e))) return r;
+    const int mx[6] = {0, 0, 0, 0, 0, 0};
+    double s[6];
+    if ((r = c->reduce(6, mx, [&] { return c->be->bulk(c->L(), c->g, c->U[u], c->E[e], c->part); }, s))) return r;
+    for (int k = 0; k < 6; k++) out[k] = s[k]/(double)c->g.V;
+    return TCYM_OK;
+}
+int tcym_gauss(tcym_ctx* c, int u, int e, double* smax, double* rms) {
+    if (!c) return TCYM_E_ARG;
+    int r;
+    if ((r = c->useU(u)) || (r = c->useE(e))) return r;
+    const int mx[2] = {1, 0};
+    double s[2];
+    if ((r = c->reduce(2, mx, [&] { return c->be->gauss(c->L(), c->g, c->U[u], c->E[e], c->part, nullptr); }, s))) return r;
+    if (smax) *smax = s[0];
+    if (rms) *rms = sqrt(s[1]/((double)c->be->ne*c->g.V));
+    return TCYM_OK;
+}
+int tcym_unitarity(tcym_ctx* c, int u, double* umax, double* rms) {
+    if (!c) return TCYM_E_ARG;
+    int r = c->useU(u);
+    if (r) return r;
+    const int mx[2] = {1, 0};
+    double s[2];
+    if ((r = c->reduce(2, mx, [&] { return c->be->unitarity(c->L(), c->g, c->U[u], c->part); }, s))) return r;
+    if (umax) *umax = s[0];
+    if (rms) *rms = sqrt(s[1]/(3.0*c->g.V));
+    return TCYM_OK;
+}
+int tcym_gauss_restore(tcym_ctx* c, int u, int e, int maxiter, double tol, int* iters, double* final_max) {
+    if (!c) return TCYM_E_ARG;
+    int r;
+    if ((r = c->useU(u)) || (r = c->useE(e))) return r;
+    if ((r = c->alloc((void**)&c->G64, sizeof(double)*(size_t)(c->g.V*c->be->ne), "hipMalloc(G)"))) return r;
+    double* Uw;
+    double* Ew;
+    if (c->prec == TCYM_FP32) {
+        if ((r = c->alloc((void**)&c->U64, sizeof(double)*(size_t)c->link_elems(), "hipMalloc(U64)"))) return r;
+        if ((r = c->alloc((void**)&c->E64, sizeof(double)*(size_t)c->field_elems(), "hipMalloc(E64)"))) return r;
+        c->be->to_f64(c->L(), c->link_elems(), c->U[u], c->U64);
+        c->be->to_f64(c->L(), c->field_elems(), c->E[e], c->E64);
+        Uw = c->U64; Ew = c->E64;
+    } else {
+        Uw = (double*)c->U[u]; Ew = (double*)c->E[e];
+    }
+    const int mx[2] = {1, 0};
+    double s[2];
+    auto dev = [&] { return c->be64->gauss(c->L(), c->g, Uw, Ew, c->part, c->G64); };
+    if ((r = c->reduce(2, mx, dev, s))) return r;
+    double smax = s[0];
+    int n = 0;
+    const double gam[3] = {7.0/24.0, 7.0/48.0, 7.0/72.0};   // GAUSSLAWFIXING/GaussRestore.cpp:152-154
+    while (n < maxiter && smax > tol) {
+        for (int k = 0; k < 3; k++) {
+            if (k < 2) dev();
+            else if ((r = c->reduce(2, mx, dev, s))) return r;
+            c->be64->gauss_correct(c->L(), c->g, Uw, Ew, c->G64, gam[k]);
+        }
+        smax = s[0];
+        n++;
+        if (c->last != hipSuccess) return TCYM_E_HIP;
+    }
+    if (c->prec == TCYM_FP32) c->be->from_f64(c->L(), c->field_elems(), c->E64, c->E[e]);
+    if ((r = c->sync())) return r;
+    if (iters) *iters = n;
+    if (final_max) *final_max = smax;
+    if (smax > tol) return c->fail(TCYM_E_NOCONV, "Gauss-law restoration reached its iteration cap");
+    return TCYM_OK;
+}
+int tcym_geodesic(tcym_ctx* c, int uold, int unew, int umid, int emid, double cc) {
+    if (!c) return TCYM_E_ARG;
+    int r;
+    if ((r = c->useU(uold)) || (r = c->useU(unew)) || (r = c->useU(umid)) || (r = c->useE(emid))) return r;
+    c->be->geodesic(c->L(), c->g, cc, c->U[uold], c->U[unew], c->U[umid], c->E[emid]);
+    return c->sync();
+}
+int tcym_delta_ncs_cooling(tcym_ctx* c, int uold, int unew, double* out) {
+    if (!c || !out) return TCYM_E_ARG;
+    int r = tcym_geodesic(c, uold, unew, TCYM_U_MID, TCYM_E_MID, 0.5);
+    if (r) return r;
+    double v = 0.0, w = 0.0, x;
+    // CoolingMethod::GetDeltaNCs(1): Simpson (1,4,1)/6 (TOPOLOGY/CoolingMethod.cpp:80-121)
+    if ((r = c->ncsdot(TCYM_U_MID, TCYM_E_MID, &x))) return r;
+    v += 4.0*x; w += 4.0;
+    if ((r = c->ncsdot(uold, TCYM_E_MID, &x))) return r;
+    v += x; w += 1.0;
+    if ((r = c->ncsdot(unew, TCYM_E_MID, &x))) return r;
+    v += x; w += 1.0;
+    *out = v/w;
+    return TCYM_OK;
+}
+int tcym_cooling_start(tcym_ctx* c, int u_hot, int nsteps, double dtau) {
+    if (!c) return TCYM_E_ARG;
+    int r = tcym_cool(c, u_hot, TCYM_U_NEW, nsteps, dtau);
+    if (r) return r;
+    return tcym_copy_links(c, TCYM_U_OLD, TCYM_U_NEW);
+}
+int tcym_cooling_update(tcym_ctx* c, int u_hot, int nsteps, double dtau, int measure, double* dncs) {
+    if (!c) return TCYM_E_ARG;
+    int r = tcym_cool(c, u_hot, TCYM_U_NEW, nsteps, dtau);
+    if (r) return r;
+    if (measure) {
+        double d;
+        if ((r = tcym_delta_ncs_cooling(c, TCYM_U_OLD, TCYM_U_NEW, &d))) return r;
+        if (dncs) *dncs = d;
+    }
+    return tcym_copy_links(c, TCYM_U_OLD, TCYM_U_NEW);
+}
+int tcym_vacuum_ncs(tcym_ctx* c, int u, double* out) {
+    if (!c || !out) return TCYM_E_ARG;
+    int r = c->useU(u);
+    if (r) return r;
+    const int mx[1] = {0};
+    return c->reduce(1, mx, [&] { return c->be->vacuum_ncs(c->L(), c->g, c->U[u], c->part); }, out);
+}
+
+// ---------------------------------------------------------------- timing
+static int resolve(tcym_ctx* c) {
+    int r = c->sync();
+    if (r) return r;
+    for (auto& q : c->pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, q.a, q.b);
+        auto& s = c->stats[q.name];
+        s.first += ms;
+        s.second += 1;
+        c->free_events.push_back(q.a);
+        c->free_events.push_back(q.b);
+    }
+    c->pending.clear();
+    return TCYM_OK;
+}
+int tcym_timing(tcym_ctx* c, int enable) {
+    if (!c) return TCYM_E_ARG;
+    int r = resolve(c);
+    c->timing = enable != 0;
+    return r;
+}
+int tcym_timing_reset(tcym_ctx* c) {
+    if (!c) return TCYM_E_ARG;
+    int r = resolve(c);
+    c->stats.clear();
+    return r;
+}
+int tcym_kernel_time(tcym_ctx* c, const char* name, double* total_ms, long long* launches) {
+    if (!c || !name) return TCYM_E_ARG;
+    int r = resolve(c);
+    if (r) return r;
+    auto it = c->stats.find(name);
+    if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == c->stats.end() ? 0 : it->second.second;
+    return TCYM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Internal interface between the C-ABI context (tcym_api.hip) and the
+// group/precision-specialised kernel launchers (tcym_kernels_su{2,3}.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <functional>
+
+namespace tcym {
+
+struct Geom {
+    int n[3];
+    long V;            // sites
+    double a[3];       // lattice spacings
+    double aScale;
+};
+
+// one kernel launch; the context wraps it with optional HIP-event timing
+using Launch = std::function<void(hipStream_t)>;
+using Launcher = std::function<void(const char* name, const Launch&)>;
+
+struct Backend {
+    int ns = 0, ne = 0, elsize = 0, nc = 0;
+    virtual ~Backend() {}
+    // layout conversions: host AoS FP64 (staged on the device) <-> device SoA
+    virtual void pack_links(const Launcher&, const Geom&, const double* aos, void* soa) = 0;
+    virtual void unpack_links(const Launcher&, const Geom&, const void* soa, double* aos) = 0;
+    virtual void pack_efield(const Launcher&, const Geom&, const double* aos, void* soa) = 0;
+    virtual void unpack_efield(const Launcher&, const Geom&, const void* soa, double* aos) = 0;
+    virtual void to_f64(const Launcher&, long n, const void* src, double* dst) = 0;
+    virtual void copy(const Launcher&, long n, const void* src, void* dst) = 0;   // n elements of the storage type
+    virtual void from_f64(const Launcher&, long n, const double* src, void* dst) = 0;
+    // seeded fields
+    virtual void random_links(const Launcher&, const Geom&, uint64_t seed, double sigma, void* U) = 0;
+    virtual void random_efield(const Launcher&, const Geom&, uint64_t seed, uint32_t stream, uint32_t step, double sigma, void* E) = 0;
+    // updates; Uin -> Uout (distinct buffers), E in place
+    // mode: 0 leapfrog, 1 Langevin (Philox noise), 2 Langevin (Xi field), 3 flow (E := force)
+    virtual void update(const Launcher&, const Geom&, int mode, bool links, bool write_e, double dt,
+                        double damping, double noise_sd, uint64_t seed, uint32_t step,
+                        const void* Uin, void* Uout, void* E, const void* Xi) = 0;
+    // U <- exp(i cU_mu E) U (Uin -> Uout)
+    virtual void link_update(const Launcher&, const Geom&, const double* cU, const void* Uin, void* Uout, const void* E) = 0;
+    // reductions: per-block partials (double) written to `part`; return number of blocks
+    virtual long edotb(const Launcher&, const Geom&, int improved, const void* U, const void* E, double* part, double* map) = 0;
+    virtual long bulk(const Launcher&, const Geom&, const void* U, const void* E, double* part) = 0;
+    virtual long gauss(const Launcher&, const Geom&, const void* U, const void* E, double* part, double* G) = 0;
+    virtual long unitarity(const Launcher&, const Geom&, const void* U, double* part) = 0;
+    virtual long vacuum_ncs(const Launcher&, const Geom&, const void* U, double* part) = 0;
+    // Gauss-law restoration (FP64 buffers only)
+    virtual void gauss_correct(const Launcher&, const Geom&, const double* U, double* E, const double* G, double gamma) = 0;
+    virtual void geodesic(const Launcher&, const Geom&, double c, const void* Uold, const void* Unew, void* Umid, void* Emid) = 0;
+    virtual void reunitarize(const Launcher&, const Geom&, void* U) = 0;
+};
+
+Backend* make_backend_su2(int fp32);
+Backend* make_backend_su3(int fp32);
+
+// final deterministic reduction of nblk x nq partials -> out[nq] (sum or max per quantity)
+void final_reduce(const Launcher&, const double* part, long nblk, int nq, const int* is_max, double* out_dev);
+
+constexpr int kBlock = 256;
+
+}  // namespace tcym

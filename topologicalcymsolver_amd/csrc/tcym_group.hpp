@@ -1,0 +1,504 @@
+// Device-side SU(2)/SU(3) algebra for the gfx950 kernels.
+//
+// Storage conventions are those of the reference's GaugeLinks/ElectricFields:
+//   SU(2): 4 reals (u0,u1,u2,u3), U = u3 + i u_a sigma_a   (SUNc/SU2/GroupOperations.cpp:4-9)
+//   SU(3): 9 complex, column-major (i,j) -> i+3j, stored re,im (SUNc/SU3/GroupOperations.cpp:4-14)
+//   algebra: t^a = sigma_a/2 or lambda_a/2 (Gell-Mann order)
+// Every routine is templated on the arithmetic type T (double for the FP64
+// form, float for the FP32 form); reunitarisation always runs in double.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tcym {
+
+// 1/k!, k = 0..16
+__device__ constexpr double kInvFact[17] = {
+    1.0, 1.0, 0.5, 1.0/6.0, 1.0/24.0, 1.0/120.0, 1.0/720.0, 1.0/5040.0, 1.0/40320.0,
+    1.0/362880.0, 1.0/3628800.0, 1.0/39916800.0, 1.0/479001600.0, 1.0/6227020800.0,
+    1.0/87178291200.0, 1.0/1307674368000.0, 1.0/20922789888000.0};
+
+template <typename T> struct Lit;
+template <> struct Lit<double> { static constexpr int exp_terms = 15; };
+template <> struct Lit<float>  { static constexpr int exp_terms = 8; };
+
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float  dsqrt(float x)  { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double dsin(double x) { return sin(x); }
+__device__ __forceinline__ float  dsin(float x)  { return sinf(x); }
+__device__ __forceinline__ double dcos(double x) { return cos(x); }
+__device__ __forceinline__ float  dcos(float x)  { return cosf(x); }
+
+// ============================================================== SU(2)
+template <typename T> struct SU2 {
+    static constexpr int NS = 4, NE = 3, NC = 2;
+    template <typename U> using rebind = SU2<U>;
+    struct Mat { T v[4]; };
+
+    __device__ static __forceinline__ Mat unit() { Mat m; m.v[0] = m.v[1] = m.v[2] = T(0); m.v[3] = T(1); return m; }
+
+    // (a3 + i a.s)(b3 + i b.s) = (a3b3 - a.b) + i(a3 b + b3 a - a x b).s
+    __device__ static __forceinline__ Mat mul(const Mat& A, const Mat& B) {
+        Mat C;
+        C.v[0] = A.v[3]*B.v[0] + B.v[3]*A.v[0] - (A.v[1]*B.v[2] - A.v[2]*B.v[1]);
+        C.v[1] = A.v[3]*B.v[1] + B.v[3]*A.v[1] - (A.v[2]*B.v[0] - A.v[0]*B.v[2]);
+        C.v[2] = A.v[3]*B.v[2] + B.v[3]*A.v[2] - (A.v[0]*B.v[1] - A.v[1]*B.v[0]);
+        C.v[3] = A.v[3]*B.v[3] - A.v[0]*B.v[0] - A.v[1]*B.v[1] - A.v[2]*B.v[2];
+        return C;
+    }
+    __device__ static __forceinline__ Mat dag(const Mat& A) { Mat D; D.v[0] = -A.v[0]; D.v[1] = -A.v[1]; D.v[2] = -A.v[2]; D.v[3] = A.v[3]; return D; }
+    __device__ static __forceinline__ Mat mul_nd(const Mat& A, const Mat& B) { return mul(A, dag(B)); }
+    __device__ static __forceinline__ Mat mul_dn(const Mat& A, const Mat& B) { return mul(dag(A), B); }
+    __device__ static __forceinline__ Mat mul_dd(const Mat& A, const Mat& B) { return mul(dag(A), dag(B)); }
+    __device__ static __forceinline__ void axpy(T s, const Mat& X, Mat& Y) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) Y.v[k] += s*X.v[k];
+    }
+    // o[a] = c ReTr(i t^a M) = -c m_a
+    __device__ static __forceinline__ void proj(T c, const Mat& M, T* o) { o[0] = -c*M.v[0]; o[1] = -c*M.v[1]; o[2] = -c*M.v[2]; }
+    __device__ static __forceinline__ T retr_1m(const Mat& M) { return T(2)*(T(1) - M.v[3]); }
+
+    // exp(i c al^a sigma_a/2) in closed form
+    __device__ static __forceinline__ Mat expi(T c, const T* al) {
+        T x0 = c*al[0], x1 = c*al[1], x2 = c*al[2];
+        T d = dsqrt(x0*x0 + x1*x1 + x2*x2);
+        Mat U;
+        if (d != T(0)) {
+            T h = T(0.5)*d, m = c*dsin(h)/d;
+            U.v[0] = m*al[0]; U.v[1] = m*al[1]; U.v[2] = m*al[2]; U.v[3] = dcos(h);
+        } else U = unit();
+        return U;
+    }
+    // al with exp(i c al t) = U, principal branch (atan2)
+    __device__ static __forceinline__ void logi(T c, const Mat& U, T* al) {
+        T d = dsqrt(U.v[0]*U.v[0] + U.v[1]*U.v[1] + U.v[2]*U.v[2]);
+        if (d != T(0)) {
+            T m = T(2)*atan2(d, U.v[3])/(c*d);
+            al[0] = m*U.v[0]; al[1] = m*U.v[1]; al[2] = m*U.v[2];
+        } else al[0] = al[1] = al[2] = T(0);
+    }
+    // (V E V^+)^a = (w^2-|v|^2)E + 2v(v.E) + 2w(E x v)
+    __device__ static __forceinline__ void adj_act(const Mat& V, const T* E, T* o) {
+        T w = V.v[3], vv = V.v[0]*V.v[0] + V.v[1]*V.v[1] + V.v[2]*V.v[2];
+        T ve = V.v[0]*E[0] + V.v[1]*E[1] + V.v[2]*E[2], s = w*w - vv;
+        T x0 = E[1]*V.v[2] - E[2]*V.v[1], x1 = E[2]*V.v[0] - E[0]*V.v[2], x2 = E[0]*V.v[1] - E[1]*V.v[0];
+        o[0] = s*E[0] + T(2)*V.v[0]*ve + T(2)*w*x0;
+        o[1] = s*E[1] + T(2)*V.v[1]*ve + T(2)*w*x1;
+        o[2] = s*E[2] + T(2)*V.v[2]*ve + T(2)*w*x2;
+    }
+    __device__ static __forceinline__ void inv_adj_act(const Mat& V, const T* E, T* o) { adj_act(dag(V), E, o); }
+    __device__ static __forceinline__ T unitarity_norm(const Mat& U) {
+        Mat C = mul_nd(U, U);
+        return dsqrt(C.v[0]*C.v[0] + C.v[1]*C.v[1] + C.v[2]*C.v[2] + (C.v[3] - T(1))*(C.v[3] - T(1)));
+    }
+    // FP64 normalisation of the quaternion
+    __device__ static __forceinline__ void reunitarize(Mat& U) {
+        double a = U.v[0], b = U.v[1], c = U.v[2], d = U.v[3];
+        double n = 1.0/__builtin_sqrt(a*a + b*b + c*c + d*d);
+        U.v[0] = T(a*n); U.v[1] = T(b*n); U.v[2] = T(c*n); U.v[3] = T(d*n);
+    }
+    __device__ static __forceinline__ T fstruct(int a, int b, int c) { return T(((a-b)*(a-c)*(c-b))/2); }
+};
+
+// ============================================================== SU(3)
+template <typename T> struct SU3 {
+    static constexpr int NS = 18, NE = 8, NC = 3;
+    template <typename U> using rebind = SU3<U>;
+    struct Mat { T r[9], i[9]; };   // element (row, col) at row + 3 col
+
+    __device__ static __forceinline__ Mat unit() {
+        Mat m;
+#pragma unroll
+        for (int k = 0; k < 9; k++) { m.r[k] = T(0); m.i[k] = T(0); }
+        m.r[0] = m.r[4] = m.r[8] = T(1);
+        return m;
+    }
+    // C = op(A) op(B); dA/dB select the hermitian conjugate
+    template <bool dA, bool dB>
+    __device__ static __forceinline__ Mat mulg(const Mat& A, const Mat& B) {
+        Mat C;
+#pragma unroll
+        for (int row = 0; row < 3; row++) {
+#pragma unroll
+            for (int col = 0; col < 3; col++) {
+                T sr = T(0), si = T(0);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int ia = dA ? (k + 3*row) : (row + 3*k);
+                    const int ib = dB ? (col + 3*k) : (k + 3*col);
+                    const T ar = A.r[ia], ai = dA ? -A.i[ia] : A.i[ia];
+                    const T br = B.r[ib], bi = dB ? -B.i[ib] : B.i[ib];
+                    sr = __builtin_fma(ar, br, sr); sr = __builtin_fma(-ai, bi, sr);
+                    si = __builtin_fma(ar, bi, si); si = __builtin_fma(ai, br, si);
+                }
+                C.r[row + 3*col] = sr; C.i[row + 3*col] = si;
+            }
+        }
+        return C;
+    }
+    __device__ static __forceinline__ Mat mul(const Mat& A, const Mat& B) { return mulg<false, false>(A, B); }
+    __device__ static __forceinline__ Mat mul_nd(const Mat& A, const Mat& B) { return mulg<false, true>(A, B); }
+    __device__ static __forceinline__ Mat mul_dn(const Mat& A, const Mat& B) { return mulg<true, false>(A, B); }
+    __device__ static __forceinline__ Mat mul_dd(const Mat& A, const Mat& B) { return mulg<true, true>(A, B); }
+    __device__ static __forceinline__ Mat dag(const Mat& A) {
+        Mat D;
+#pragma unroll
+        for (int row = 0; row < 3; row++)
+#pragma unroll
+            for (int col = 0; col < 3; col++) { D.r[row + 3*col] = A.r[col + 3*row]; D.i[row + 3*col] = -A.i[col + 3*row]; }
+        return D;
+    }
+    __device__ static __forceinline__ void axpy(T s, const Mat& X, Mat& Y) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) { Y.r[k] += s*X.r[k]; Y.i[k] += s*X.i[k]; }
+    }
+    // o[a] = c ReTr(i t^a M), Gell-Mann order
+    __device__ static __forceinline__ void proj(T c, const Mat& M, T* o) {
+        const T h = T(-0.5)*c, s3 = T(0.57735026918962576450914878050196);
+        o[0] = h*(M.i[3] + M.i[1]);
+        o[1] = h*(M.r[3] - M.r[1]);
+        o[2] = h*(M.i[0] - M.i[4]);
+        o[3] = h*(M.i[6] + M.i[2]);
+        o[4] = h*(M.r[6] - M.r[2]);
+        o[5] = h*(M.i[7] + M.i[5]);
+        o[6] = h*(M.r[7] - M.r[5]);
+        o[7] = h*(M.i[0] + M.i[4] - T(2)*M.i[8])*s3;
+    }
+    __device__ static __forceinline__ T retr_1m(const Mat& M) { return T(3) - M.r[0] - M.r[4] - M.r[8]; }
+
+    // hermitian H = al^a t^a
+    __device__ static __forceinline__ Mat alg_matrix(T c, const T* al) {
+        const T s3 = T(0.57735026918962576450914878050196), h = T(0.5)*c;
+        Mat A;
+        A.r[0] = h*(al[2] + al[7]*s3);  A.i[0] = T(0);
+        A.r[4] = -h*(al[2] - al[7]*s3); A.i[4] = T(0);
+        A.r[8] = -c*al[7]*s3;           A.i[8] = T(0);
+        A.r[1] = h*al[0]; A.i[1] = h*al[1];     // (1,0)
+        A.r[2] = h*al[3]; A.i[2] = h*al[4];     // (2,0)
+        A.r[5] = h*al[5]; A.i[5] = h*al[6];     // (2,1)
+        A.r[3] = A.r[1]; A.i[3] = -A.i[1];      // (0,1)
+        A.r[6] = A.r[2]; A.i[6] = -A.i[2];      // (0,2)
+        A.r[7] = A.r[5]; A.i[7] = -A.i[5];      // (1,2)
+        return A;
+    }
+    // o^a = tr(lambda_a X) for hermitian X
+    __device__ static __forceinline__ void herm_alg(const Mat& X, T* o) {
+        const T s3 = T(0.57735026918962576450914878050196);
+        o[0] = X.r[3] + X.r[1];
+        o[1] = X.i[1] - X.i[3];
+        o[2] = X.r[0] - X.r[4];
+        o[3] = X.r[6] + X.r[2];
+        o[4] = X.i[2] - X.i[6];
+        o[5] = X.r[7] + X.r[5];
+        o[6] = X.i[5] - X.i[7];
+        o[7] = (X.r[0] + X.r[4] - T(2)*X.r[8])*s3;
+    }
+    __device__ static __forceinline__ void adj_act(const Mat& V, const T* E, T* o) {
+        Mat A = alg_matrix(T(1), E);
+        Mat X = mul_nd(mul(V, A), V);
+        herm_alg(X, o);
+    }
+    __device__ static __forceinline__ void inv_adj_act(const Mat& V, const T* E, T* o) {
+        Mat A = alg_matrix(T(1), E);
+        Mat X = mul(mul_dn(V, A), V);
+        herm_alg(X, o);
+    }
+
+    // exp(i Q), Q = c al^a t^a, as f0 + f1 Q + f2 Q^2 (Cayley-Hamilton):
+    // Q^3 = c1 Q + c0 with c1 = tr(Q^2)/2, c0 = det Q.  The Taylor series of
+    // exp(iQ) is summed in the (1, Q, Q^2) basis with scalar recurrences, after
+    // scaling Q by 2^-s so that c1 <= 1/4; the result is squared s times in
+    // the same basis.
+    __device__ static __forceinline__ Mat expi(T c, const T* al) {
+        Mat Q = alg_matrix(c, al);
+        T c1 = T(0);
+#pragma unroll
+        for (int k = 0; k < 9; k++) c1 += Q.r[k]*Q.r[k] + Q.i[k]*Q.i[k];
+        c1 *= T(0.5);
+        if (c1 == T(0)) return unit();
+        // det of hermitian Q (real)
+        T m0 = Q.r[4]*Q.r[8] - (Q.r[7]*Q.r[7] + Q.i[7]*Q.i[7]);          // Q11 Q22 - |Q12|^2
+        // Q01*(Q12 Q20 - Q10 Q22) + Q02*(Q10 Q21 - Q11 Q20), real part
+        T ar = Q.r[7]*Q.r[2] - Q.i[7]*Q.i[2] - Q.r[1]*Q.r[8];
+        T ai = Q.r[7]*Q.i[2] + Q.i[7]*Q.r[2] - Q.i[1]*Q.r[8];
+        T br = Q.r[1]*Q.r[5] - Q.i[1]*Q.i[5] - Q.r[4]*Q.r[2];
+        T bi = Q.r[1]*Q.i[5] + Q.i[1]*Q.r[5] - Q.r[4]*Q.i[2];
+        T c0 = Q.r[0]*m0 + (Q.r[3]*ar - Q.i[3]*ai) + (Q.r[6]*br - Q.i[6]*bi);
+        int s = 0;
+        T cs1 = c1, cs0 = c0, sc = T(1);
+        while (cs1 > T(0.25) && s < 40) { cs1 *= T(0.25); cs0 *= T(0.125); sc *= T(0.5); s++; }
+        // Taylor series in the (1,Q,Q^2) basis
+        T pa = T(1), pb = T(0), pd = T(0);             // Q^k = pa + pb Q + pd Q^2
+        T f0r = T(0), f0i = T(0), f1r = T(0), f1i = T(0), f2r = T(0), f2i = T(0);
+#pragma unroll
+        for (int k = 0; k <= Lit<T>::exp_terms; k++) {
+            const T fact = T(kInvFact[k]);
+            const T w = ((k >> 1) & 1) ? -fact : fact;   // i^k = (+1, +i, -1, -i)
+            if (k & 1) { f0i += w*pa; f1i += w*pb; f2i += w*pd; }
+            else       { f0r += w*pa; f1r += w*pb; f2r += w*pd; }
+            const T na = cs0*pd, nb = pa + cs1*pd, nd = pb;
+            pa = na; pb = nb; pd = nd;
+        }
+        for (int k = 0; k < s; k++) {
+            // p^2 reduced with Q^3 = cs1 Q + cs0, Q^4 = cs1 Q^2 + cs0 Q
+            T a0r = f0r*f0r - f0i*f0i, a0i = T(2)*f0r*f0i;
+            T m12r = f1r*f2r - f1i*f2i, m12i = f1r*f2i + f1i*f2r;
+            T m22r = f2r*f2r - f2i*f2i, m22i = T(2)*f2r*f2i;
+            T m01r = f0r*f1r - f0i*f1i, m01i = f0r*f1i + f0i*f1r;
+            T m02r = f0r*f2r - f0i*f2i, m02i = f0r*f2i + f0i*f2r;
+            T m11r = f1r*f1r - f1i*f1i, m11i = T(2)*f1r*f1i;
+            T n0r = a0r + T(2)*m12r*cs0, n0i = a0i + T(2)*m12i*cs0;
+            T n1r = T(2)*m01r + T(2)*m12r*cs1 + m22r*cs0, n1i = T(2)*m01i + T(2)*m12i*cs1 + m22i*cs0;
+            T n2r = T(2)*m02r + m11r + m22r*cs1, n2i = T(2)*m02i + m11i + m22i*cs1;
+            f0r = n0r; f0i = n0i; f1r = n1r; f1i = n1i; f2r = n2r; f2i = n2i;
+        }
+        // U = f0 + f1 Qs + f2 Qs^2, Qs = sc Q
+#pragma unroll
+        for (int k = 0; k < 9; k++) { Q.r[k] *= sc; Q.i[k] *= sc; }
+        Mat Q2 = mul(Q, Q);
+        Mat U;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            U.r[k] = f1r*Q.r[k] - f1i*Q.i[k] + f2r*Q2.r[k] - f2i*Q2.i[k];
+            U.i[k] = f1r*Q.i[k] + f1i*Q.r[k] + f2r*Q2.i[k] + f2i*Q2.r[k];
+        }
+        U.r[0] += f0r; U.i[0] += f0i; U.r[4] += f0r; U.i[4] += f0i; U.r[8] += f0r; U.i[8] += f0i;
+        return U;
+    }
+
+    __device__ static __forceinline__ T unitarity_norm(const Mat& U) {
+        Mat C = mul_nd(U, U);
+        T n = T(0);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            T d = C.r[k] - ((k == 0 || k == 4 || k == 8) ? T(1) : T(0));
+            n += d*d + C.i[k]*C.i[k];
+        }
+        return dsqrt(n);
+    }
+    // FP64 Gram-Schmidt on rows 0,1; row 2 = conj(row0 x row1)
+    __device__ static __forceinline__ void reunitarize(Mat& U) {
+        double r0r[3], r0i[3], r1r[3], r1i[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) { r0r[j] = U.r[3*j]; r0i[j] = U.i[3*j]; r1r[j] = U.r[1 + 3*j]; r1i[j] = U.i[1 + 3*j]; }
+        double n = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) n += r0r[j]*r0r[j] + r0i[j]*r0i[j];
+        n = 1.0/__builtin_sqrt(n);
+#pragma unroll
+        for (int j = 0; j < 3; j++) { r0r[j] *= n; r0i[j] *= n; }
+        double pr = 0, pi = 0;   // <r0, r1>
+#pragma unroll
+        for (int j = 0; j < 3; j++) { pr += r0r[j]*r1r[j] + r0i[j]*r1i[j]; pi += r0r[j]*r1i[j] - r0i[j]*r1r[j]; }
+        n = 0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            r1r[j] -= pr*r0r[j] - pi*r0i[j];
+            r1i[j] -= pr*r0i[j] + pi*r0r[j];
+            n += r1r[j]*r1r[j] + r1i[j]*r1i[j];
+        }
+        n = 1.0/__builtin_sqrt(n);
+#pragma unroll
+        for (int j = 0; j < 3; j++) { r1r[j] *= n; r1i[j] *= n; }
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int a = (j + 1) % 3, b = (j + 2) % 3;
+            double cr = r0r[a]*r1r[b] - r0i[a]*r1i[b] - (r0r[b]*r1r[a] - r0i[b]*r1i[a]);
+            double ci = r0r[a]*r1i[b] + r0i[a]*r1r[b] - (r0r[b]*r1i[a] + r0i[b]*r1r[a]);
+            U.r[3*j] = T(r0r[j]); U.i[3*j] = T(r0i[j]);
+            U.r[1 + 3*j] = T(r1r[j]); U.i[1 + 3*j] = T(r1i[j]);
+            U.r[2 + 3*j] = T(cr); U.i[2 + 3*j] = T(-ci);
+        }
+    }
+    __device__ static T fstruct(int a, int b, int c);
+};
+
+// ---------------------------------------------------------------- SU(3) log (FP64)
+// -i log W for W in SU(3): eigenvectors from a complex Jacobi diagonalisation
+// of the hermitian Cayley transform C = i(1 - W')(1 + W')^-1 of W' = e^{-i s} W
+// (the shift s keeps every eigenvalue of W' away from -1), eigen-phases
+// phi = 2 atan(c) + s wrapped to the principal branch (-pi, pi], then
+// projected onto the generators.
+struct Log3 {
+    struct C { double re, im; };
+    __device__ static __forceinline__ C cm(C a, C b) { return {a.re*b.re - a.im*b.im, a.re*b.im + a.im*b.re}; }
+    __device__ static __forceinline__ C cs(C a, C b) { return {a.re - b.re, a.im - b.im}; }
+    __device__ static __forceinline__ C ca(C a, C b) { return {a.re + b.re, a.im + b.im}; }
+    __device__ static __forceinline__ C cj(C a) { return {a.re, -a.im}; }
+    __device__ static __forceinline__ C csc(double s, C a) { return {s*a.re, s*a.im}; }
+
+    __device__ static void mm(const C* A, const C* B, C* R, bool dA) {
+        C T[9];
+        for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) {
+            C s = {0, 0};
+            for (int k = 0; k < 3; k++) s = ca(s, cm(dA ? cj(A[k + 3*r]) : A[r + 3*k], B[k + 3*c]));
+            T[r + 3*c] = s;
+        }
+        for (int k = 0; k < 9; k++) R[k] = T[k];
+    }
+    __device__ static C det3(const C* M) {
+        C c00 = cs(cm(M[4], M[8]), cm(M[7], M[5]));
+        C c01 = cs(cm(M[7], M[2]), cm(M[1], M[8]));
+        C c02 = cs(cm(M[1], M[5]), cm(M[4], M[2]));
+        return ca(ca(cm(M[0], c00), cm(M[3], c01)), cm(M[6], c02));
+    }
+    __device__ static void inv3(const C* M, C* I) {
+        C c00 = cs(cm(M[4], M[8]), cm(M[7], M[5]));
+        C c01 = cs(cm(M[7], M[2]), cm(M[1], M[8]));
+        C c02 = cs(cm(M[1], M[5]), cm(M[4], M[2]));
+        C det = ca(ca(cm(M[0], c00), cm(M[3], c01)), cm(M[6], c02));
+        double dd = det.re*det.re + det.im*det.im;
+        C id = csc(1.0/dd, cj(det));
+        C c10 = cs(cm(M[6], M[5]), cm(M[3], M[8]));
+        C c11 = cs(cm(M[0], M[8]), cm(M[6], M[2]));
+        C c12 = cs(cm(M[3], M[2]), cm(M[0], M[5]));
+        C c20 = cs(cm(M[3], M[7]), cm(M[6], M[4]));
+        C c21 = cs(cm(M[6], M[1]), cm(M[0], M[7]));
+        C c22 = cs(cm(M[0], M[4]), cm(M[3], M[1]));
+        I[0] = cm(c00, id); I[3] = cm(c10, id); I[6] = cm(c20, id);
+        I[1] = cm(c01, id); I[4] = cm(c11, id); I[7] = cm(c21, id);
+        I[2] = cm(c02, id); I[5] = cm(c12, id); I[8] = cm(c22, id);
+    }
+    // hermitian Jacobi; A overwritten, V eigenvectors (columns), w eigenvalues
+    __device__ static void jacobi(C* A, C* V, double* w) {
+        for (int k = 0; k < 9; k++) V[k] = {0, 0};
+        V[0] = V[4] = V[8] = {1, 0};
+        for (int sweep = 0; sweep < 12; sweep++) {
+            double off = A[3].re*A[3].re + A[3].im*A[3].im + A[6].re*A[6].re + A[6].im*A[6].im + A[7].re*A[7].re + A[7].im*A[7].im;
+            double dia = A[0].re*A[0].re + A[4].re*A[4].re + A[8].re*A[8].re;
+            if (off == 0.0 || off < 1e-36*dia) break;
+            for (int pq = 0; pq < 3; pq++) {
+                const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+                C apq = A[p + 3*q];
+                double g = __builtin_sqrt(apq.re*apq.re + apq.im*apq.im);
+                if (g == 0.0) continue;
+                C e = csc(1.0/g, apq);
+                double th = (A[q + 3*q].re - A[p + 3*p].re)/(2.0*g);
+                double t = (th >= 0 ? 1.0 : -1.0)/(fabs(th) + __builtin_sqrt(th*th + 1.0));
+                double cc = 1.0/__builtin_sqrt(t*t + 1.0), sn = t*cc;
+                C R[9];
+                for (int k = 0; k < 9; k++) R[k] = {0, 0};
+                for (int k = 0; k < 3; k++) R[k + 3*k] = {1, 0};
+                R[p + 3*p] = {cc, 0};
+                R[p + 3*q] = {sn, 0};
+                R[q + 3*p] = csc(-sn, cj(e));
+                R[q + 3*q] = csc(cc, cj(e));
+                C T[9];
+                mm(A, R, T, false);
+                mm(R, T, A, true);
+                mm(V, R, T, false);
+                for (int k = 0; k < 9; k++) V[k] = T[k];
+                A[p + 3*q] = A[q + 3*p] = {0, 0};
+                for (int k = 0; k < 3; k++) A[k + 3*k].im = 0.0;
+            }
+        }
+        for (int k = 0; k < 3; k++) w[k] = A[k + 3*k].re;
+    }
+    // W given as re/im arrays (row + 3 col); out al with exp(i c al t) = W
+    __device__ static void logi(double c, const double* Wr, const double* Wi, double* al) {
+        C W[9];
+        for (int k = 0; k < 9; k++) W[k] = {Wr[k], Wi[k]};
+        const double shifts[4] = {0.0, 1.5707963267948966, -1.5707963267948966, 3.141592653589793};
+        double best = -1.0, sft = 0.0;
+        for (int k = 0; k < 4; k++) {
+            C ph = {cos(shifts[k]), -sin(shifts[k])}, M[9];
+            for (int j = 0; j < 9; j++) M[j] = cm(ph, W[j]);
+            M[0].re += 1; M[4].re += 1; M[8].re += 1;
+            C d = det3(M);
+            double ad = d.re*d.re + d.im*d.im;
+            if (ad > best) { best = ad; sft = shifts[k]; }
+            if (k == 0 && ad > 0.25) break;
+        }
+        C ph = {cos(sft), -sin(sft)}, P[9], Mi[9];
+        for (int j = 0; j < 9; j++) { C wp = cm(ph, W[j]); P[j] = wp; Mi[j] = csc(-1.0, wp); }
+        for (int j = 0; j < 3; j++) { P[j + 3*j].re += 1; Mi[j + 3*j].re += 1; }
+        C Pinv[9], Cm[9], Ch[9];
+        inv3(P, Pinv);
+        mm(Mi, Pinv, Cm, false);
+        for (int r = 0; r < 3; r++) for (int cc = 0; cc < 3; cc++) {
+            C a = cm({0, 1}, Cm[r + 3*cc]), b = cj(cm({0, 1}, Cm[cc + 3*r]));
+            Ch[r + 3*cc] = csc(0.5, ca(a, b));
+        }
+        C V[9]; double w[3];
+        jacobi(Ch, V, w);
+        double Hr[9], Hi[9];
+        for (int k = 0; k < 9; k++) { Hr[k] = 0; Hi[k] = 0; }
+        for (int j = 0; j < 3; j++) {
+            double phi = 2.0*atan(w[j]) + sft;
+            while (phi > 3.141592653589793) phi -= 6.283185307179586;
+            while (phi <= -3.141592653589793) phi += 6.283185307179586;
+            for (int r = 0; r < 3; r++) for (int cc = 0; cc < 3; cc++) {
+                C v = cm(V[r + 3*j], cj(V[cc + 3*j]));
+                Hr[r + 3*cc] += phi*v.re; Hi[r + 3*cc] += phi*v.im;
+            }
+        }
+        const double s3 = 0.57735026918962576450914878050196;
+        al[0] = (Hr[3] + Hr[1])/c;
+        al[1] = (Hi[1] - Hi[3])/c;
+        al[2] = (Hr[0] - Hr[4])/c;
+        al[3] = (Hr[6] + Hr[2])/c;
+        al[4] = (Hi[2] - Hi[6])/c;
+        al[5] = (Hr[7] + Hr[5])/c;
+        al[6] = (Hi[5] - Hi[7])/c;
+        al[7] = (Hr[0] + Hr[4] - 2.0*Hr[8])*s3/c;
+    }
+};
+
+// su(3) structure constants, 0-based Gell-Mann indices
+template <typename T>
+__device__ T SU3<T>::fstruct(int a, int b, int c) {
+    // sort (a,b,c) keeping track of the permutation sign
+    int s = 1;
+    if (a > b) { int t = a; a = b; b = t; s = -s; }
+    if (b > c) { int t = b; b = c; c = t; s = -s; }
+    if (a > b) { int t = a; a = b; b = t; s = -s; }
+    T v = T(0);
+    const int key = a*64 + b*8 + c;
+    switch (key) {
+        case 0*64 + 1*8 + 2: v = T(1); break;
+        case 0*64 + 3*8 + 6: v = T(0.5); break;
+        case 0*64 + 4*8 + 5: v = T(-0.5); break;
+        case 1*64 + 3*8 + 5: v = T(0.5); break;
+        case 1*64 + 4*8 + 6: v = T(0.5); break;
+        case 2*64 + 3*8 + 4: v = T(0.5); break;
+        case 2*64 + 5*8 + 6: v = T(-0.5); break;
+        case 3*64 + 4*8 + 7: v = T(0.86602540378443864676372317075294); break;
+        case 5*64 + 6*8 + 7: v = T(0.86602540378443864676372317075294); break;
+        default: v = T(0);
+    }
+    return s*v;
+}
+
+// ---------------------------------------------------------------- Philox-4x32-10
+struct Philox {
+    __device__ static __forceinline__ void round4(uint32_t* c, const uint32_t* k) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u*c[0], p1 = (uint64_t)0xCD9E8D57u*c[2];
+        const uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0, h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
+        const uint32_t n0 = h1 ^ c[1] ^ k[0], n2 = h0 ^ c[3] ^ k[1];
+        c[0] = n0; c[1] = l1; c[2] = n2; c[3] = l0;
+    }
+    __device__ static __forceinline__ void gen(uint32_t* c, uint32_t k0, uint32_t k1) {
+        uint32_t k[2] = {k0, k1};
+#pragma unroll
+        for (int r = 0; r < 10; r++) { round4(c, k); k[0] += 0x9E3779B9u; k[1] += 0xBB67AE85u; }
+    }
+    __device__ static __forceinline__ double u53(uint32_t hi, uint32_t lo) {
+        const uint64_t x = ((uint64_t)hi << 21) | (uint64_t)(lo >> 11);
+        return ((double)x + 1.0)*(1.0/9007199254740992.0);
+    }
+    // n normals for (seed, stream, step, link): counter (link, step, stream, j)
+    __device__ static __forceinline__ void normals(uint64_t seed, uint32_t stream, uint32_t step, uint32_t link, int n, double* z) {
+        for (int j = 0; 2*j < n; j++) {
+            uint32_t c[4] = {link, step, stream, (uint32_t)j};
+            gen(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+            const double u1 = u53(c[0], c[1]), u2 = u53(c[2], c[3]);
+            const double R = __builtin_sqrt(-2.0*log(u1)), t = 6.283185307179586476925286766559*u2;
+            double sn, cs;
+            sincos(t, &sn, &cs);
+            z[2*j] = R*cs;
+            if (2*j + 1 < n) z[2*j + 1] = R*sn;
+        }
+    }
+};
+
+}  // namespace tcym
