@@ -47,7 +47,16 @@ def make_initial_condition(T, nc, N, seed, su, se, maxiter):
     g.random_links(seed, su)
     g.random_efield(seed, se)
     t0 = time.time()
-    it, fm = g.gauss_law_restore(maxiter=maxiter, tol=1e-14, check=False)
+    # GaussRestore in chunks of 1000 iterations so that the run reports its
+    # progress; a chunk restarts from the current field, so the stop can come
+    # at most one iteration earlier than in a single call
+    it, fm = 0, float("inf")
+    while it < maxiter:
+        n, fm = g.gauss_law_restore(maxiter=min(1000, maxiter - it), tol=1e-14, check=False)
+        it += n
+        log("# GaussRestore: %d iterations, signed max %.3e (%.1f s)" % (it, fm, time.time() - t0))
+        if fm <= 1e-14:
+            break
     log("# initial condition: GaussRestore %d iterations, signed max %.3e (%.1f s)" % (it, fm, time.time() - t0))
     U, E = g.get_links(), g.get_efield()
     g.close()
@@ -85,6 +94,8 @@ def run_form(T, nc, N, precision, U, E, steps, warmup, dt, cool_every, cool_step
             ncs += d
             dncs_series.append(ncs)
             energies.append(g.energy())
+            if k % (4*cool_every) == 0:
+                log("#   step %d: T00 %.10e  Delta N_CS %.6e  (%.1f s)" % (k, energies[-1], ncs, time.perf_counter() - t0))
     g.sync()
     wall = time.perf_counter() - t0
     stats = {}

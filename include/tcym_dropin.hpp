@@ -1,0 +1,429 @@
+// tcym_dropin.hpp -- the reference's hot-path entry points, with their
+// original namespaces, names and signatures, over GaugeLinks/ElectricFields
+// host containers in the reference's own memory layout, forwarding to the
+// MI355X library through the C ABI of tcym.h.
+//
+// A program written against TopologicalCYMSolver's
+//   Dynamics::Update(GaugeLinks*, ElectricFields*)           DYNAMICS/Dynamics.cpp:204
+//   LangevinDynamics::UpdateFields(GaugeLinks*, ElectricFields*)  DYNAMICS/LangevinDynamics.cpp:300
+//   GradientFlow::StandardUpdate / CalibrationUpdate           GRADIENTFLOW/GradientFlow.cpp:150,168
+//   Cooling::CoolNSave                                         GRADIENTFLOW/Cooling.cpp:156
+//   ChernSimonsNumber::NCsDot                                  TOPOLOGY/ChernSimonsDerivative.cpp:31
+//   ChernSimonsNumber::CoolingMethod::{Start,Update}           TOPOLOGY/CoolingMethod.cpp:127,143
+//   ImprovedOperators::ComputeEDotB                            CALC/ChernSimonsOperators.cpp:384
+//   Observables::{Bulk,GaussLaw,Unitarity}, EdotBMap::CreateMap OBSERVABLES/*.cpp
+//   GaussLawRestoration::Restore                               GAUSSLAWFIXING/GaussRestore.cpp:131
+// includes this header instead of those sources and links libtcym.so.
+//
+// Group and precision are run-time values (tcym_dropin::Configure) instead of
+// the SU_Nc_FLAG macro.  Step sizes that the reference hard-codes
+// (Dynamics::dTau, LangevinDynamics::dTauL, GradientFlow::SqrtDcTauOverSpacing)
+// are plain variables here with the reference's default values.
+//
+// Residency: by default every call uploads the host fields it reads and
+// downloads the ones it writes, so host arrays are always current.  Inside a
+// tcym_dropin::Resident scope the fields stay on the device between calls and
+// are downloaded when the scope ends (or by tcym_dropin::SyncToHost()).
+// Failures throw tcym_dropin::Error instead of calling exit(0).
+#pragma once
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <fstream>
+
+#include "tcym.h"
+
+typedef int INT;
+typedef double DOUBLE;
+
+namespace tcym_dropin {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+struct Config {
+    int nc = 3;
+    int precision = TCYM_FP64;
+    int device = 0;
+};
+inline Config& config() { static Config c; return c; }
+inline void Configure(int nc, int precision = TCYM_FP64, int device = 0) {
+    config().nc = nc; config().precision = precision; config().device = device;
+}
+inline int LinkSize() { return config().nc == 2 ? 4 : 18; }
+inline int VectorSize() { return config().nc == 2 ? 3 : 8; }
+
+}  // namespace tcym_dropin
+
+// ---------------------------------------------------------------- containers
+// Same members and index functions as LATTICE/GaugeLinks.cpp:36-114 (spacings
+// kept as DOUBLE rather than truncated to INT) and LATTICE/ElectricFields.cpp:9-88.
+class GaugeLinks {
+public:
+    INT Volume;
+    static const INT Dimension = 3;
+    DOUBLE a[Dimension];
+    DOUBLE aCube;
+    INT N[Dimension];
+    DOUBLE* U;
+    INT Index3D(INT x, INT y, INT z) const {
+        return md(x, N[0]) + N[0]*(md(y, N[1]) + N[1]*md(z, N[2]));
+    }
+    INT Index(INT x, INT y, INT z, INT mu) const { return tcym_dropin::LinkSize()*(mu + Dimension*Index3D(x, y, z)); }
+    DOUBLE* Get(INT x, INT y, INT z, INT mu) { return &U[Index(x, y, z, mu)]; }
+    GaugeLinks(INT Nx, INT Ny, INT Nz, DOUBLE ax, DOUBLE ay, DOUBLE az) {
+        Volume = Nx*Ny*Nz; N[0] = Nx; N[1] = Ny; N[2] = Nz;
+        a[0] = ax; a[1] = ay; a[2] = az; aCube = ax*ay*az;
+        U = new DOUBLE[(size_t)Dimension*tcym_dropin::LinkSize()*Volume]();
+    }
+    ~GaugeLinks() { delete[] U; }
+    GaugeLinks(const GaugeLinks&) = delete;
+    GaugeLinks& operator=(const GaugeLinks&) = delete;
+private:
+    static INT md(INT x, INT n) { return x >= 0 ? x % n : (x + n) % n; }
+};
+
+class ElectricFields {
+public:
+    INT Volume;
+    static const INT Dimension = 3;
+    DOUBLE a[Dimension];
+    DOUBLE aCube;
+    INT N[Dimension];
+    DOUBLE* E;
+    INT Index3D(INT x, INT y, INT z) const {
+        return md(x, N[0]) + N[0]*(md(y, N[1]) + N[1]*md(z, N[2]));
+    }
+    INT Index(INT x, INT y, INT z, INT mu, INT a_) const { return a_ + tcym_dropin::VectorSize()*(mu + Dimension*Index3D(x, y, z)); }
+    DOUBLE* Get(INT x, INT y, INT z, INT mu, INT a_) { return &E[Index(x, y, z, mu, a_)]; }
+    void SetZero() { std::memset(E, 0, sizeof(DOUBLE)*(size_t)Dimension*tcym_dropin::VectorSize()*Volume); }
+    ElectricFields(INT Nx, INT Ny, INT Nz, DOUBLE ax, DOUBLE ay, DOUBLE az) {
+        Volume = Nx*Ny*Nz; N[0] = Nx; N[1] = Ny; N[2] = Nz;
+        a[0] = ax; a[1] = ay; a[2] = az; aCube = ax*ay*az;
+        E = new DOUBLE[(size_t)Dimension*tcym_dropin::VectorSize()*Volume]();
+    }
+    ~ElectricFields() { delete[] E; }
+    ElectricFields(const ElectricFields&) = delete;
+    ElectricFields& operator=(const ElectricFields&) = delete;
+private:
+    static INT md(INT x, INT n) { return x >= 0 ? x % n : (x + n) % n; }
+};
+
+namespace Lattice {
+inline DOUBLE& aScale() { static DOUBLE s = 1.0; return s; }   // LATTICE/3DGrid.cpp:18
+}
+
+namespace tcym_dropin {
+
+// One device context per lattice geometry.  Host objects are bound to device
+// slots; a binding remembers whether the device copy is newer than the host.
+class Device {
+public:
+    static Device& get(const INT* N, const DOUBLE* a) {
+        static std::map<std::string, Device*> all;
+        char key[160];
+        snprintf(key, sizeof key, "%d %d %d %d %d %.17g %.17g %.17g %.17g", config().nc, config().precision,
+                 N[0], N[1], N[2], a[0], a[1], a[2], Lattice::aScale());
+        auto it = all.find(key);
+        if (it != all.end()) return *it->second;
+        Device* d = new Device(N, a);
+        all[key] = d;
+        return *d;
+    }
+    tcym_ctx* ctx = nullptr;
+    void check(int r) const {
+        if (r != TCYM_OK) throw Error(r, std::string("tcym: ") + tcym_last_error(ctx));
+    }
+    // bind host links to a device slot.  Resident: reuse any slot that holds a
+    // current copy of the object; otherwise upload into `slot` (first saving a
+    // device-newer object that the slot held).
+    int links(GaugeLinks* U, int slot) {
+        if (resident)
+            for (int s = 0; s < TCYM_NUM_U; s++) if (ub[s].host == U && ub[s].valid) return s;
+        flush_links(slot);
+        for (int s = 0; s < TCYM_NUM_U; s++) if (ub[s].host == U) { flush_links(s); ub[s] = Bind(); }
+        check(tcym_set_links(ctx, slot, U->U));
+        ub[slot].host = U; ub[slot].valid = true; ub[slot].dirty = false;
+        return slot;
+    }
+    int field(ElectricFields* E, int slot) {
+        if (resident)
+            for (int s = 0; s < TCYM_NUM_E; s++) if (eb[s].host == E && eb[s].valid) return s;
+        flush_field(slot);
+        for (int s = 0; s < TCYM_NUM_E; s++) if (eb[s].host == E) { flush_field(s); eb[s] = Bind(); }
+        check(tcym_set_efield(ctx, slot, E->E));
+        eb[slot].host = E; eb[slot].valid = true; eb[slot].dirty = false;
+        return slot;
+    }
+    // the device copy was modified: download now unless resident
+    void wrote_links(int slot) {
+        ub[slot].dirty = true;
+        if (!resident) flush_links(slot);
+    }
+    void wrote_field(int slot) {
+        eb[slot].dirty = true;
+        if (!resident) flush_field(slot);
+    }
+    void flush() {
+        for (int s = 0; s < TCYM_NUM_U; s++) flush_links(s);
+        for (int s = 0; s < TCYM_NUM_E; s++) flush_field(s);
+    }
+    void invalidate() {
+        for (auto& b : ub) b.valid = false;
+        for (auto& b : eb) b.valid = false;
+    }
+    static bool& resident_flag() { static bool r = false; return r; }
+    bool resident = false;
+private:
+    struct Bind { void* host = nullptr; bool valid = false, dirty = false; };
+    Bind ub[TCYM_NUM_U], eb[TCYM_NUM_E];
+    void flush_links(int s) {
+        if (ub[s].dirty && ub[s].host) check(tcym_get_links(ctx, s, ((GaugeLinks*)ub[s].host)->U));
+        ub[s].dirty = false;
+    }
+    void flush_field(int s) {
+        if (eb[s].dirty && eb[s].host) check(tcym_get_efield(ctx, s, ((ElectricFields*)eb[s].host)->E));
+        eb[s].dirty = false;
+    }
+    Device(const INT* N, const DOUBLE* a) {
+        tcym_params p;
+        p.nc = config().nc; p.precision = config().precision; p.device = config().device;
+        for (int k = 0; k < 3; k++) { p.N[k] = N[k]; p.a[k] = a[k]; }
+        p.aScale = Lattice::aScale();
+        int r = tcym_create(&p, &ctx);
+        if (r != TCYM_OK) throw Error(r, "tcym_create failed");
+    }
+};
+
+inline Device& dev(const GaugeLinks* U) { Device& d = Device::get(U->N, U->a); d.resident = Device::resident_flag(); return d; }
+
+// RAII residency scope: fields stay on the device until the scope ends
+struct Resident {
+    Resident() { Device::resident_flag() = true; }
+    ~Resident() { Device::resident_flag() = false; }
+};
+// download every device-newer field of the lattice's context to its host object
+inline void SyncToHost(GaugeLinks* U) { Device& d = Device::get(U->N, U->a); d.flush(); }
+// host arrays were modified by the caller: re-upload on next use
+inline void HostModified(GaugeLinks* U) { Device::get(U->N, U->a).invalidate(); }
+
+}  // namespace tcym_dropin
+
+// ---------------------------------------------------------------- dynamics
+namespace Dynamics {
+inline DOUBLE& dTau() { static DOUBLE v = 0.025; return v; }   // DYNAMICS/Dynamics.cpp:17
+inline INT& tSteps() { static INT v = 0; return v; }
+inline DOUBLE Time(DOUBLE Qs = 1.0) { return Qs*(tSteps()*dTau()); }   // :20-22
+inline void Reset() { tSteps() = 0; }
+inline void UpdateElectricFields(INT, INT, INT, INT, INT, INT, GaugeLinks* U, ElectricFields* E) {
+    auto& d = tcym_dropin::dev(U);
+    const int u = d.links(U, TCYM_U_HOT), e = d.field(E, TCYM_E_HOT);
+    d.check(tcym_update_efield(d.ctx, u, e, dTau()));
+    d.wrote_field(e);
+}
+inline void UpdateGaugeLinks(INT, INT, INT, INT, INT, INT, GaugeLinks* U, ElectricFields* E) {
+    auto& d = tcym_dropin::dev(U);
+    const int u = d.links(U, TCYM_U_HOT), e = d.field(E, TCYM_E_HOT);
+    d.check(tcym_update_links(d.ctx, u, e, dTau()));
+    d.wrote_links(u);
+}
+inline void Update(GaugeLinks* U, ElectricFields* E) {   // DYNAMICS/Dynamics.cpp:204-218
+    auto& d = tcym_dropin::dev(U);
+    const int u = d.links(U, TCYM_U_HOT), e = d.field(E, TCYM_E_HOT);
+    d.check(tcym_leapfrog(d.ctx, u, e, 1, dTau()));
+    d.wrote_links(u);
+    d.wrote_field(e);
+    tSteps()++;
+}
+}  // namespace Dynamics
+
+namespace LangevinDynamics {
+inline DOUBLE& dTauL() { static DOUBLE v = 0.001; return v; }   // DYNAMICS/LangevinDynamics.cpp:41
+inline DOUBLE& beta() { static DOUBLE v = 2.0; return v; }      // ThermalDynamics::beta (DYNAMICS/ThermalDynamics.cpp:10)
+inline DOUBLE& damping() { static DOUBLE v = 1.0; return v; }   // fixed at 1 in the reference
+inline INT& tauLSteps() { static INT v = 0; return v; }
+inline uint64_t& seed() { static uint64_t v = 1; return v; }
+// LangevinDynamics::UpdateFields with LangevinVariables::Xi (DYNAMICS/LangevinDynamics.cpp:300-311)
+inline void UpdateFields(GaugeLinks* U, ElectricFields* E, ElectricFields* Xi) {
+    auto& d = tcym_dropin::dev(U);
+    const int u = d.links(U, TCYM_U_HOT), e = d.field(E, TCYM_E_HOT), x = d.field(Xi, TCYM_E_XI);
+    d.check(tcym_langevin_xi(d.ctx, u, e, x, dTauL(), damping()));
+    d.wrote_links(u);
+    d.wrote_field(e);
+    tauLSteps()++;
+}
+// LangevinDynamics::Update: Philox noise of variance 2 damping/beta, then UpdateFields (:320-347)
+inline void Update(GaugeLinks* U, ElectricFields* E) {
+    auto& d = tcym_dropin::dev(U);
+    const int u = d.links(U, TCYM_U_HOT), e = d.field(E, TCYM_E_HOT);
+    d.check(tcym_langevin(d.ctx, u, e, 1, dTauL(), damping(), 1.0/beta(), seed(), (uint32_t)tauLSteps()));
+    d.wrote_links(u);
+    d.wrote_field(e);
+    tauLSteps()++;
+}
+}  // namespace LangevinDynamics
+
+namespace ChernSimonsNumber {
+inline DOUBLE& DeltaNCsCooling() { static DOUBLE v = 0.0; return v; }        // TOPOLOGY/ChernSimonsDerivative.cpp:18
+inline DOUBLE& DeltaNCsCoolRealTime() { static DOUBLE v = 0.0; return v; }   // :15
+inline DOUBLE NCsDot(GaugeLinks* U, ElectricFields* E) {                      // :31-49
+    auto& d = tcym_dropin::dev(U);
+    double v;
+    d.check(tcym_ncsdot(d.ctx, d.links(U, TCYM_U_AUX), d.field(E, TCYM_E_AUX), &v));
+    return v;
+}
+}  // namespace ChernSimonsNumber
+
+namespace ImprovedOperators {
+inline DOUBLE ComputeEDotB(GaugeLinks* U, ElectricFields* E) {   // CALC/ChernSimonsOperators.cpp:384-388
+    auto& d = tcym_dropin::dev(U);
+    double v;
+    d.check(tcym_edotb(d.ctx, d.links(U, TCYM_U_AUX), d.field(E, TCYM_E_AUX), 1, &v));
+    return v;
+}
+}  // namespace ImprovedOperators
+
+namespace UnimprovedOperators {
+inline DOUBLE ComputeEDotB(GaugeLinks* U, ElectricFields* E) {   // CALC/ChernSimonsOperators.cpp:49-53
+    auto& d = tcym_dropin::dev(U);
+    double v;
+    d.check(tcym_edotb(d.ctx, d.links(U, TCYM_U_AUX), d.field(E, TCYM_E_AUX), 0, &v));
+    return v;
+}
+}  // namespace UnimprovedOperators
+
+namespace GradientFlow {
+inline DOUBLE& SqrtDcTauOverSpacing() { static DOUBLE v = 0.5/std::sqrt(2.0); return v; }   // GRADIENTFLOW/GradientFlow.cpp:15
+inline DOUBLE& cTau() { static DOUBLE v = 0.0; return v; }
+inline INT& CoolingSteps() { static INT v = 0; return v; }
+inline void Reset() { cTau() = 0.0; CoolingSteps() = 0; }
+inline DOUBLE CoolingTime() { return cTau(); }
+inline DOUBLE dtau() { return SqrtDcTauOverSpacing()*SqrtDcTauOverSpacing(); }
+inline void advance(const GaugeLinks* U) {
+    cTau() += dtau()*std::pow(U->aCube, 2.0/3.0)/(Lattice::aScale()*Lattice::aScale());
+    CoolingSteps()++;
+}
+inline void StandardUpdate(GaugeLinks* U, ElectricFields* E) {   // :150-165
+    auto& d = tcym_dropin::dev(U);
+    const int u = d.links(U, TCYM_U_COOL), e = d.field(E, TCYM_E_COOL);
+    d.check(tcym_flow(d.ctx, u, e, 1, dtau(), 0, nullptr));
+    d.wrote_links(u);
+    d.wrote_field(e);
+    advance(U);
+}
+inline void CalibrationUpdate(GaugeLinks* U, ElectricFields* E) {   // :168-188
+    auto& d = tcym_dropin::dev(U);
+    double dn = 0.0;
+    const int u = d.links(U, TCYM_U_COOL), e = d.field(E, TCYM_E_COOL);
+    d.check(tcym_flow(d.ctx, u, e, 1, dtau(), 1, &dn));
+    ChernSimonsNumber::DeltaNCsCooling() += dn;
+    d.wrote_links(u);
+    d.wrote_field(e);
+    advance(U);
+}
+}  // namespace GradientFlow
+
+namespace Cooling {
+// Cooling::CoolNSave without blocking (GRADIENTFLOW/Cooling.cpp:156-179)
+inline void CoolNSave(std::string, DOUBLE MaxCoolingSteps, INT MaxBlockingNumber, GaugeLinks* UHot, GaugeLinks* UCold) {
+    if (MaxBlockingNumber != 0) throw tcym_dropin::Error(TCYM_E_UNSUPPORTED, "blocking of cooled copies is not provided");
+    auto& d = tcym_dropin::dev(UHot);
+    GradientFlow::Reset();
+    d.check(tcym_cool(d.ctx, d.links(UHot, TCYM_U_HOT), TCYM_U_NEW, (int)MaxCoolingSteps, GradientFlow::dtau()));
+    for (int k = 0; k < (int)MaxCoolingSteps; k++) GradientFlow::advance(UHot);
+    d.check(tcym_get_links(d.ctx, TCYM_U_NEW, UCold->U));
+}
+}  // namespace Cooling
+
+namespace ChernSimonsNumber {
+namespace CoolingMethod {
+inline INT& StandardCoolingMaxSteps() { static INT v = 128; return v; }   // SphaleronSimulation.cpp:230
+// CoolingMethod::Start (TOPOLOGY/CoolingMethod.cpp:127-137): cools GLinks::U into UNew, UOld <- UNew
+inline void Start(GaugeLinks* UHot) {
+    auto& d = tcym_dropin::dev(UHot);
+    d.check(tcym_cooling_start(d.ctx, d.links(UHot, TCYM_U_HOT), StandardCoolingMaxSteps(), GradientFlow::dtau()));
+}
+// CoolingMethod::Update(Measure) (:143-165)
+inline void Update(GaugeLinks* UHot, INT Measure) {
+    auto& d = tcym_dropin::dev(UHot);
+    DeltaNCsCooling() = 0.0;
+    double dn = 0.0;
+    d.check(tcym_cooling_update(d.ctx, d.links(UHot, TCYM_U_HOT), StandardCoolingMaxSteps(), GradientFlow::dtau(),
+                                Measure ? 1 : 0, &dn));
+    if (Measure == 1) DeltaNCsCoolRealTime() += dn;
+}
+}  // namespace CoolingMethod
+}  // namespace ChernSimonsNumber
+
+namespace Observables {
+namespace Bulk {
+inline DOUBLE* ESqr() { static DOUBLE v[3]; return v; }
+inline DOUBLE* BSqr() { static DOUBLE v[3]; return v; }
+inline DOUBLE* cE() { static DOUBLE v[3]; return v; }
+inline DOUBLE* cB() { static DOUBLE v[3]; return v; }
+inline void Update(GaugeLinks* U, ElectricFields* E) {   // OBSERVABLES/BulkObservables.cpp:133-143
+    auto& d = tcym_dropin::dev(U);
+    double o[6];
+    d.check(tcym_bulk(d.ctx, d.links(U, TCYM_U_AUX), d.field(E, TCYM_E_AUX), o));
+    for (int mu = 0; mu < 3; mu++) {
+        ESqr()[mu] = o[mu]; BSqr()[mu] = o[3 + mu];
+        const double c = U->a[mu]*Lattice::aScale()*Lattice::aScale()/U->aCube;   // :17-25 (Minkowski)
+        cE()[mu] = cB()[mu] = c*c;
+    }
+}
+inline DOUBLE T00() { return 0.5*(cE()[0]*ESqr()[0] + cE()[1]*ESqr()[1] + cE()[2]*ESqr()[2] + cB()[0]*BSqr()[0] + cB()[1]*BSqr()[1] + cB()[2]*BSqr()[2]); }
+inline DOUBLE ELECTRIC() { return 0.5*(cE()[0]*ESqr()[0] + cE()[1]*ESqr()[1] + cE()[2]*ESqr()[2]); }
+inline DOUBLE MAGNETIC() { return 0.5*(cB()[0]*BSqr()[0] + cB()[1]*BSqr()[1] + cB()[2]*BSqr()[2]); }
+}  // namespace Bulk
+namespace GaussLaw {
+inline DOUBLE& GlobalMaxViolation() { static DOUBLE v; return v; }
+inline DOUBLE& GlobalAvgViolation() { static DOUBLE v; return v; }
+inline void UpdateViolation(INT, INT, INT, INT, INT, INT, GaugeLinks* U, ElectricFields* E) {   // OBSERVABLES/GaussLawViolation.cpp:13-61
+    auto& d = tcym_dropin::dev(U);
+    d.check(tcym_gauss(d.ctx, d.links(U, TCYM_U_AUX), d.field(E, TCYM_E_AUX), &GlobalMaxViolation(), &GlobalAvgViolation()));
+}
+}  // namespace GaussLaw
+namespace Unitarity {
+inline DOUBLE& GlobalMaxViolation() { static DOUBLE v; return v; }
+inline DOUBLE& GlobalAvgViolation() { static DOUBLE v; return v; }
+inline void UpdateViolation(INT, INT, INT, INT, INT, INT, GaugeLinks* U) {   // OBSERVABLES/UnitarityViolation.cpp:13-51
+    auto& d = tcym_dropin::dev(U);
+    d.check(tcym_unitarity(d.ctx, d.links(U, TCYM_U_AUX), &GlobalMaxViolation(), &GlobalAvgViolation()));
+}
+}  // namespace Unitarity
+namespace EdotBMap {
+// writes "x y z EdotB" per site (OBSERVABLES/EdotBMap.cpp:20-348, output :323)
+inline void CreateMap(std::string fname, GaugeLinks* U, ElectricFields* E) {
+    auto& d = tcym_dropin::dev(U);
+    double* m = new double[(size_t)U->Volume];
+    d.check(tcym_edotb_map(d.ctx, d.links(U, TCYM_U_AUX), d.field(E, TCYM_E_AUX), m));
+    std::ofstream out(fname.c_str());
+    out.precision(15);
+    for (INT z = 0; z < U->N[2]; z++)
+        for (INT y = 0; y < U->N[1]; y++)
+            for (INT x = 0; x < U->N[0]; x++) out << x << " " << y << " " << z << " " << m[U->Index3D(x, y, z)] << "\n";
+    delete[] m;
+}
+}  // namespace EdotBMap
+}  // namespace Observables
+
+namespace GaussLawRestoration {
+inline DOUBLE& GlobalMaxViolation() { static DOUBLE v; return v; }
+inline INT& Iterations() { static INT v; return v; }
+inline void Restore(GaugeLinks* U, ElectricFields* E) {   // GAUSSLAWFIXING/GaussRestore.cpp:131-167
+    auto& d = tcym_dropin::dev(U);
+    int it = 0;
+    double fm = 0.0;
+    const int u = d.links(U, TCYM_U_HOT), e = d.field(E, TCYM_E_HOT);
+    int r = tcym_gauss_restore(d.ctx, u, e, 10000, 1e-14, &it, &fm);
+    if (r != TCYM_OK && r != TCYM_E_NOCONV) d.check(r);
+    Iterations() = it;
+    GlobalMaxViolation() = fm;
+    d.wrote_field(e);
+}
+}  // namespace GaussLawRestoration
