@@ -124,8 +124,9 @@ namespace tcym_dropin {
 // slots; a binding remembers whether the device copy is newer than the host.
 class Device {
 public:
+    static std::map<std::string, Device*>& registry() { static std::map<std::string, Device*> m; return m; }
     static Device& get(const INT* N, const DOUBLE* a) {
-        static std::map<std::string, Device*> all;
+        std::map<std::string, Device*>& all = registry();
         char key[160];
         snprintf(key, sizeof key, "%d %d %d %d %d %.17g %.17g %.17g %.17g", config().nc, config().precision,
                  N[0], N[1], N[2], a[0], a[1], a[2], Lattice::aScale());
@@ -202,10 +203,15 @@ private:
 
 inline Device& dev(const GaugeLinks* U) { Device& d = Device::get(U->N, U->a); d.resident = Device::resident_flag(); return d; }
 
-// RAII residency scope: fields stay on the device until the scope ends
+// RAII residency scope: fields stay on the device until the scope ends, then
+// every device-newer field is downloaded to its host object (which must still
+// exist at that point)
 struct Resident {
     Resident() { Device::resident_flag() = true; }
-    ~Resident() { Device::resident_flag() = false; }
+    ~Resident() {
+        for (auto& kv : Device::registry()) kv.second->flush();
+        Device::resident_flag() = false;
+    }
 };
 // download every device-newer field of the lattice's context to its host object
 inline void SyncToHost(GaugeLinks* U) { Device& d = Device::get(U->N, U->a); d.flush(); }

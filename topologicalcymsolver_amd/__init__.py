@@ -14,7 +14,7 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libtcym.so")
+LIB_PATH = os.environ.get("TCYM_LIBRARY") or os.path.join(PKG, "libtcym.so")
 
 OK, E_ARG, E_HIP, E_NOCONV, E_NONFINITE, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 FP64, FP32 = 0, 1

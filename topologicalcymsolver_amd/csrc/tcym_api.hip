@@ -206,6 +206,13 @@ int tcym_create(const tcym_params* p, tcym_ctx** out) {
     c->g.aScale = p->aScale;
     c->g.V = (long)p->N[0]*p->N[1]*p->N[2];
     const int f32 = p->precision == TCYM_FP32;
+    {
+        // the kernels address each field through one buffer descriptor with
+        // 32-bit offsets: every field (3 V s_U reals) must stay below 4 GiB
+        const double V = (double)p->N[0]*p->N[1]*p->N[2];
+        const double bytes = 3.0*V*(p->nc == 2 ? 4 : 18)*(f32 ? 4 : 8);
+        if (bytes >= 4294967296.0) return TCYM_E_UNSUPPORTED;
+    }
     c->be = p->nc == 2 ? tcym::make_backend_su2(f32) : tcym::make_backend_su3(f32);
     c->be64 = f32 ? (p->nc == 2 ? tcym::make_backend_su2(0) : tcym::make_backend_su3(0)) : c->be;
     int r = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");

@@ -22,6 +22,10 @@ template <typename T> struct Lit;
 template <> struct Lit<double> { static constexpr int exp_terms = 15; };
 template <> struct Lit<float>  { static constexpr int exp_terms = 8; };
 
+// fused multiply-add in the arithmetic type itself (no promotion of FP32 to FP64)
+__device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float  fmat(float a, float b, float c)  { return __builtin_fmaf(a, b, c); }
+
 __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float  dsqrt(float x)  { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double dsin(double x) { return sin(x); }
@@ -128,8 +132,8 @@ template <typename T> struct SU3 {
                     const int ib = dB ? (col + 3*k) : (k + 3*col);
                     const T ar = A.r[ia], ai = dA ? -A.i[ia] : A.i[ia];
                     const T br = B.r[ib], bi = dB ? -B.i[ib] : B.i[ib];
-                    sr = __builtin_fma(ar, br, sr); sr = __builtin_fma(-ai, bi, sr);
-                    si = __builtin_fma(ar, bi, si); si = __builtin_fma(ai, br, si);
+                    sr = fmat(ar, br, sr); sr = fmat(-ai, bi, sr);
+                    si = fmat(ar, bi, si); si = fmat(ai, br, si);
                 }
                 C.r[row + 3*col] = sr; C.i[row + 3*col] = si;
             }
