@@ -216,7 +216,7 @@ int tcym_create(const tcym_params* p, tcym_ctx** out) {
     c->be = p->nc == 2 ? tcym::make_backend_su2(f32) : tcym::make_backend_su3(f32);
     c->be64 = f32 ? (p->nc == 2 ? tcym::make_backend_su2(0) : tcym::make_backend_su3(0)) : c->be;
     int r = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-    const long nblk = (c->g.V + kBlock - 1)/kBlock;
+    const long nblk = tcym::walk_shape(c->g).blocks;
     if (!r) r = c->alloc((void**)&c->part, sizeof(double)*(size_t)nblk*8, "hipMalloc(partials)");
     if (!r) r = c->alloc((void**)&c->red, sizeof(double)*16, "hipMalloc(results)");
     if (!r) r = c->hip(hipHostMalloc((void**)&c->red_host, sizeof(double)*16, hipHostMallocDefault), "hipHostMalloc");

@@ -59,4 +59,28 @@ void final_reduce(const Launcher&, const double* part, long nblk, int nq, const 
 
 constexpr int kBlock = 256;
 
+// Traversal of the lattice by the stencil kernels.  A block covers a patch of
+// (1 << pxs) sites in x by (kBlock >> pxs) rows in y of one z-plane; the
+// patches stacked in y make a column kColRows sites high, and a column is
+// swept through all z before the next column.  pxs is 6, 5 or 4, whichever
+// pads Nx least (ties go to the wider patch, for coalescing).
+constexpr int kColRows = 16;
+struct WalkShape {
+    int pxs;           // log2 of the patch width in x
+    int tcx, tcy;      // columns in x and y
+    long blocks;       // blocks for one pass over the sites
+};
+inline WalkShape walk_shape(const Geom& g) {
+    WalkShape w{6, 0, 0, 0};
+    long best = -1;
+    for (int s = 6; s >= 4; s--) {
+        const long padded = (long)((g.n[0] + (1 << s) - 1) >> s) << s;
+        if (best < 0 || padded < best) { best = padded; w.pxs = s; }
+    }
+    w.tcx = (g.n[0] + (1 << w.pxs) - 1) >> w.pxs;
+    w.tcy = (g.n[1] + kColRows - 1)/kColRows;
+    w.blocks = (long)w.tcx*w.tcy*g.n[2]*(kColRows/(kBlock >> w.pxs));
+    return w;
+}
+
 }  // namespace tcym
