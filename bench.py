@@ -42,6 +42,26 @@ def bytes_per_link_update(nc, kind, fp32):
     raise ValueError(kind)
 
 
+def measured_traffic(nc, N, kind, fp32):
+    """HBM bytes per launch of the dominant kernel, as measured by the FETCH_SIZE
+    and WRITE_SIZE counters (calibrated against a copy kernel) in a separate
+    profiler run whose summary is kept in profiles/; None if none matches."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None, None
+    for f in sorted(os.listdir(d)):
+        if not f.endswith("_traffic.json"):
+            continue
+        try:
+            with open(os.path.join(d, f)) as fh:
+                t = json.load(fh)
+        except (OSError, ValueError):
+            continue
+        if (t.get("kernel"), t.get("dtype"), t.get("nc"), t.get("N")) == (kind, "f32" if fp32 else "f64", nc, N):
+            return t["traffic_bytes"], "profiles/" + t.get("source", f)
+    return None, None
+
+
 def make_initial_condition(T, nc, N, seed, su, se, maxiter):
     g = T.Solver(nc, N, T.FP64)
     g.random_links(seed, su)
@@ -188,6 +208,7 @@ def main():
     avg_s = tot_ms / max(launches, 1) / 1e3
     bpl = bytes_per_link_update(nc, dom, True)
     achieved = 3 * V * bpl / avg_s / 1e9
+    traffic, traffic_src = measured_traffic(nc, args.N, dom, True)
     kernels = {}
     for k, (ms, n) in r32["stats"].items():
         if n:
@@ -224,7 +245,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_algorithmic": 3 * V * bpl, "traffic_source": traffic_src,
             "bytes_per_link_update": bpl, "avg_launch_us": avg_s * 1e6,
         },
         "kernels": kernels,

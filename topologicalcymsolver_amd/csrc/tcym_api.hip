@@ -50,6 +50,12 @@ __global__ __launch_bounds__(256) void k_final(const double* __restrict__ part, 
     }
 }
 
+// a field of nr reals of `bytes` each per (site, mu) fits the 32-bit buffer offsets
+bool field_fits(const int* N, int nr, int bytes) {
+    const double V = (double)tcym::padded_sites((long)N[0]*N[1]*N[2]);
+    return 3.0*V*nr*bytes < 4294967296.0;
+}
+
 struct Pending {
     std::string name;
     hipEvent_t a, b;
@@ -90,8 +96,11 @@ struct tcym_ctx {
     std::map<std::string, std::pair<double, long long>> stats;
     hipError_t last = hipSuccess;
 
-    long link_elems() const { return 3*g.V*be->ns; }
-    long field_elems() const { return 3*g.V*be->ne; }
+    // device fields (sites padded to whole tiles) and host AoS arrays
+    long link_elems() const { return 3*tcym::padded_sites(g.V)*be->ns; }
+    long field_elems() const { return 3*tcym::padded_sites(g.V)*be->ne; }
+    long host_link_elems() const { return 3*g.V*be->ns; }
+    long host_field_elems() const { return 3*g.V*be->ne; }
 
     hipEvent_t ev() {
         if (!free_events.empty()) { hipEvent_t e = free_events.back(); free_events.pop_back(); return e; }
@@ -200,19 +209,16 @@ int tcym_create(const tcym_params* p, tcym_ctx** out) {
     if (hipGetDeviceProperties(&prop, p->device) != hipSuccess) return TCYM_E_HIP;
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return TCYM_E_UNSUPPORTED;
     if (hipSetDevice(p->device) != hipSuccess) return TCYM_E_HIP;
+    const int f32 = p->precision == TCYM_FP32;
+    // the kernels address each field through one buffer descriptor with
+    // 32-bit offsets: every field (3 V s_U reals, sites padded to tiles) must
+    // stay below 4 GiB
+    if (!field_fits(p->N, p->nc == 2 ? 4 : 18, f32 ? 4 : 8)) return TCYM_E_UNSUPPORTED;
     tcym_ctx* c = new tcym_ctx();
     c->nc = p->nc; c->prec = p->precision; c->device = p->device;
     for (int k = 0; k < 3; k++) { c->g.n[k] = p->N[k]; c->g.a[k] = p->a[k]; }
     c->g.aScale = p->aScale;
     c->g.V = (long)p->N[0]*p->N[1]*p->N[2];
-    const int f32 = p->precision == TCYM_FP32;
-    {
-        // the kernels address each field through one buffer descriptor with
-        // 32-bit offsets: every field (3 V s_U reals) must stay below 4 GiB
-        const double V = (double)p->N[0]*p->N[1]*p->N[2];
-        const double bytes = 3.0*V*(p->nc == 2 ? 4 : 18)*(f32 ? 4 : 8);
-        if (bytes >= 4294967296.0) return TCYM_E_UNSUPPORTED;
-    }
     c->be = p->nc == 2 ? tcym::make_backend_su2(f32) : tcym::make_backend_su3(f32);
     c->be64 = f32 ? (p->nc == 2 ? tcym::make_backend_su2(0) : tcym::make_backend_su3(0)) : c->be;
     int r = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -257,7 +263,7 @@ int tcym_set_links(tcym_ctx* c, int slot, const double* U) {
     if (!c || !U) return TCYM_E_ARG;
     int r = c->useU(slot);
     if (r) return r;
-    if ((r = c->hip(hipMemcpyAsync(c->stage, U, sizeof(double)*(size_t)c->link_elems(), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(links)"))) return r;
+    if ((r = c->hip(hipMemcpyAsync(c->stage, U, sizeof(double)*(size_t)c->host_link_elems(), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(links)"))) return r;
     c->be->pack_links(c->L(), c->g, c->stage, c->U[slot]);
     return c->sync();
 }
@@ -266,14 +272,14 @@ int tcym_get_links(tcym_ctx* c, int slot, double* U) {
     int r = c->useU(slot);
     if (r) return r;
     c->be->unpack_links(c->L(), c->g, c->U[slot], c->stage);
-    if ((r = c->hip(hipMemcpyAsync(U, c->stage, sizeof(double)*(size_t)c->link_elems(), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(links)"))) return r;
+    if ((r = c->hip(hipMemcpyAsync(U, c->stage, sizeof(double)*(size_t)c->host_link_elems(), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(links)"))) return r;
     return c->sync();
 }
 int tcym_set_efield(tcym_ctx* c, int slot, const double* E) {
     if (!c || !E) return TCYM_E_ARG;
     int r = c->useE(slot);
     if (r) return r;
-    if ((r = c->hip(hipMemcpyAsync(c->stage, E, sizeof(double)*(size_t)c->field_elems(), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(efield)"))) return r;
+    if ((r = c->hip(hipMemcpyAsync(c->stage, E, sizeof(double)*(size_t)c->host_field_elems(), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(efield)"))) return r;
     c->be->pack_efield(c->L(), c->g, c->stage, c->E[slot]);
     return c->sync();
 }
@@ -282,7 +288,7 @@ int tcym_get_efield(tcym_ctx* c, int slot, double* E) {
     int r = c->useE(slot);
     if (r) return r;
     c->be->unpack_efield(c->L(), c->g, c->E[slot], c->stage);
-    if ((r = c->hip(hipMemcpyAsync(E, c->stage, sizeof(double)*(size_t)c->field_elems(), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(efield)"))) return r;
+    if ((r = c->hip(hipMemcpyAsync(E, c->stage, sizeof(double)*(size_t)c->host_field_elems(), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(efield)"))) return r;
     return c->sync();
 }
 int tcym_copy_links(tcym_ctx* c, int dst, int src) {
@@ -484,10 +490,12 @@ int tcym_gauss_restore(tcym_ctx* c, int u, int e, int maxiter, double tol, int* 
     double* Uw;
     double* Ew;
     if (c->prec == TCYM_FP32) {
+        if (!field_fits(c->g.n, c->be->ns, 8))
+            return c->fail(TCYM_E_UNSUPPORTED, "tcym_gauss_restore: the FP64 copy of the links would exceed 4 GiB");
         if ((r = c->alloc((void**)&c->U64, sizeof(double)*(size_t)c->link_elems(), "hipMalloc(U64)"))) return r;
         if ((r = c->alloc((void**)&c->E64, sizeof(double)*(size_t)c->field_elems(), "hipMalloc(E64)"))) return r;
-        c->be->to_f64(c->L(), c->link_elems(), c->U[u], c->U64);
-        c->be->to_f64(c->L(), c->field_elems(), c->E[e], c->E64);
+        c->be->to_f64(c->L(), c->g, c->be->ns, c->U[u], c->U64);
+        c->be->to_f64(c->L(), c->g, c->be->ne, c->E[e], c->E64);
         Uw = c->U64; Ew = c->E64;
     } else {
         Uw = (double*)c->U[u]; Ew = (double*)c->E[e];
@@ -509,7 +517,7 @@ int tcym_gauss_restore(tcym_ctx* c, int u, int e, int maxiter, double tol, int* 
         n++;
         if (c->last != hipSuccess) return TCYM_E_HIP;
     }
-    if (c->prec == TCYM_FP32) c->be->from_f64(c->L(), c->field_elems(), c->E64, c->E[e]);
+    if (c->prec == TCYM_FP32) c->be->from_f64(c->L(), c->g, c->be->ne, c->E64, c->E[e]);
     if ((r = c->sync())) return r;
     if (iters) *iters = n;
     if (final_max) *final_max = smax;

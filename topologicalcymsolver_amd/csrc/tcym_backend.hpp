@@ -21,14 +21,16 @@ using Launcher = std::function<void(const char* name, const Launch&)>;
 struct Backend {
     int ns = 0, ne = 0, elsize = 0, nc = 0;
     virtual ~Backend() {}
-    // layout conversions: host AoS FP64 (staged on the device) <-> device SoA
+    // layout conversions: host AoS FP64 (staged on the device) <-> device tiled layout
     virtual void pack_links(const Launcher&, const Geom&, const double* aos, void* soa) = 0;
     virtual void unpack_links(const Launcher&, const Geom&, const void* soa, double* aos) = 0;
     virtual void pack_efield(const Launcher&, const Geom&, const double* aos, void* soa) = 0;
     virtual void unpack_efield(const Launcher&, const Geom&, const void* soa, double* aos) = 0;
-    virtual void to_f64(const Launcher&, long n, const void* src, double* dst) = 0;
+    // storage type <-> FP64 for fields of nr reals per (site, mu); the tiled
+    // layout depends on the storage type, so these are not element-wise
+    virtual void to_f64(const Launcher&, const Geom&, int nr, const void* src, double* dst) = 0;
     virtual void copy(const Launcher&, long n, const void* src, void* dst) = 0;   // n elements of the storage type
-    virtual void from_f64(const Launcher&, long n, const double* src, void* dst) = 0;
+    virtual void from_f64(const Launcher&, const Geom&, int nr, const double* src, void* dst) = 0;
     // seeded fields
     virtual void random_links(const Launcher&, const Geom&, uint64_t seed, double sigma, void* U) = 0;
     virtual void random_efield(const Launcher&, const Geom&, uint64_t seed, uint32_t stream, uint32_t step, double sigma, void* E) = 0;
@@ -59,12 +61,19 @@ void final_reduce(const Launcher&, const double* part, long nblk, int nq, const 
 
 constexpr int kBlock = 256;
 
+// sites of a device field: the tiled layout stores sites in tiles of 64
+inline long padded_sites(long V) { return (V + 63) & ~63L; }
+
 // Traversal of the lattice by the stencil kernels.  A block covers a patch of
 // (1 << pxs) sites in x by (kBlock >> pxs) rows in y of one z-plane; the
 // patches stacked in y make a column kColRows sites high, and a column is
 // swept through all z before the next column.  pxs is 6, 5 or 4, whichever
-// pads Nx least (ties go to the wider patch, for coalescing).
-constexpr int kColRows = 16;
+// pads Nx least among the patches that fit in a column (ties go to the wider
+// patch, for coalescing).
+#ifndef TCYM_COL_ROWS
+#define TCYM_COL_ROWS 16
+#endif
+constexpr int kColRows = TCYM_COL_ROWS;
 struct WalkShape {
     int pxs;           // log2 of the patch width in x
     int tcx, tcy;      // columns in x and y
@@ -74,6 +83,7 @@ inline WalkShape walk_shape(const Geom& g) {
     WalkShape w{6, 0, 0, 0};
     long best = -1;
     for (int s = 6; s >= 4; s--) {
+        if ((kBlock >> s) > kColRows) continue;          // a patch must fit in a column
         const long padded = (long)((g.n[0] + (1 << s) - 1) >> s) << s;
         if (best < 0 || padded < best) { best = padded; w.pxs = s; }
     }

@@ -60,6 +60,19 @@ template <typename T> struct SU2 {
     }
     // o[a] = c ReTr(i t^a M) = -c m_a
     __device__ static __forceinline__ void proj(T c, const Mat& M, T* o) { o[0] = -c*M.v[0]; o[1] = -c*M.v[1]; o[2] = -c*M.v[2]; }
+    // o[a] += c ReTr(i t^a A B): only the vector part of the product
+    __device__ static __forceinline__ void proj_mul_acc(T c, const Mat& A, const Mat& B, T* o) {
+        o[0] -= c*(A.v[3]*B.v[0] + B.v[3]*A.v[0] - (A.v[1]*B.v[2] - A.v[2]*B.v[1]));
+        o[1] -= c*(A.v[3]*B.v[1] + B.v[3]*A.v[1] - (A.v[2]*B.v[0] - A.v[0]*B.v[2]));
+        o[2] -= c*(A.v[3]*B.v[2] + B.v[3]*A.v[2] - (A.v[0]*B.v[1] - A.v[1]*B.v[0]));
+    }
+    // a A + b B
+    __device__ static __forceinline__ Mat lin(T a, const Mat& A, T b, const Mat& B) {
+        Mat C;
+#pragma unroll
+        for (int k = 0; k < 4; k++) C.v[k] = a*A.v[k] + b*B.v[k];
+        return C;
+    }
     __device__ static __forceinline__ T retr_1m(const Mat& M) { return T(2)*(T(1) - M.v[3]); }
 
     // exp(i c al^a sigma_a/2) in closed form
@@ -169,6 +182,50 @@ template <typename T> struct SU3 {
         o[7] = h*(M.i[0] + M.i[4] - T(2)*M.i[8])*s3;
     }
     __device__ static __forceinline__ T retr_1m(const Mat& M) { return T(3) - M.r[0] - M.r[4] - M.r[8]; }
+    // o[a] += c ReTr(i t^a A B), evaluating only the entries of A B the
+    // projection reads: the six off-diagonal ones and the diagonal's imaginary parts
+    __device__ static __forceinline__ void proj_mul_acc(T c, const Mat& A, const Mat& B, T* o) {
+        auto ent = [&](int row, int col, T& re, T& im) {
+            T sr = T(0), si = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int ia = row + 3*k, ib = k + 3*col;
+                sr = fmat(A.r[ia], B.r[ib], sr); sr = fmat(-A.i[ia], B.i[ib], sr);
+                si = fmat(A.r[ia], B.i[ib], si); si = fmat(A.i[ia], B.r[ib], si);
+            }
+            re = sr; im = si;
+        };
+        auto dim = [&](int d) -> T {
+            T si = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                si = fmat(A.r[d + 3*k], B.i[k + 3*d], si);
+                si = fmat(A.i[d + 3*k], B.r[k + 3*d], si);
+            }
+            return si;
+        };
+        T r10, i10, r01, i01, r20, i20, r02, i02, r21, i21, r12, i12;
+        ent(1, 0, r10, i10); ent(0, 1, r01, i01);
+        ent(2, 0, r20, i20); ent(0, 2, r02, i02);
+        ent(2, 1, r21, i21); ent(1, 2, r12, i12);
+        const T i00 = dim(0), i11 = dim(1), i22 = dim(2);
+        const T h = T(-0.5)*c, s3 = T(0.57735026918962576450914878050196);
+        o[0] += h*(i01 + i10);
+        o[1] += h*(r01 - r10);
+        o[2] += h*(i00 - i11);
+        o[3] += h*(i02 + i20);
+        o[4] += h*(r02 - r20);
+        o[5] += h*(i12 + i21);
+        o[6] += h*(r12 - r21);
+        o[7] += h*(i00 + i11 - T(2)*i22)*s3;
+    }
+    // a A + b B
+    __device__ static __forceinline__ Mat lin(T a, const Mat& A, T b, const Mat& B) {
+        Mat C;
+#pragma unroll
+        for (int k = 0; k < 9; k++) { C.r[k] = a*A.r[k] + b*B.r[k]; C.i[k] = a*A.i[k] + b*B.i[k]; }
+        return C;
+    }
 
     // hermitian H = al^a t^a
     __device__ static __forceinline__ Mat alg_matrix(T c, const T* al) {
