@@ -33,6 +33,16 @@ EXPORTS = [
     "tcym_gauss_restore", "tcym_geodesic", "tcym_delta_ncs_cooling", "tcym_vacuum_ncs",
     "tcym_timing", "tcym_kernel_time", "tcym_timing_reset", "tcym_sync", "tcym_device_bytes",
 ]
+# every exported symbol of include/tcym_batch.h
+BATCH_EXPORTS = [
+    "tcym_batch_create", "tcym_batch_destroy", "tcym_batch_last_error", "tcym_batch_sync",
+    "tcym_batch_device_bytes", "tcym_batch_set_links", "tcym_batch_get_links", "tcym_batch_set_efield",
+    "tcym_batch_get_efield", "tcym_batch_initial_condition", "tcym_batch_set_step", "tcym_batch_get_step",
+    "tcym_batch_leapfrog", "tcym_batch_langevin", "tcym_batch_flow", "tcym_batch_evolve",
+    "tcym_batch_cooling_start", "tcym_batch_cooling_update", "tcym_batch_ncsdot", "tcym_batch_bulk",
+    "tcym_batch_gauss", "tcym_batch_unitarity", "tcym_batch_tune", "tcym_batch_timing",
+    "tcym_batch_kernel_time", "tcym_batch_timing_reset",
+]
 
 
 class TcymError(RuntimeError):
@@ -44,6 +54,12 @@ class TcymError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [("nc", ctypes.c_int), ("precision", ctypes.c_int), ("N", ctypes.c_int * 3),
                 ("a", ctypes.c_double * 3), ("aScale", ctypes.c_double), ("device", ctypes.c_int)]
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [("nc", ctypes.c_int), ("precision", ctypes.c_int), ("count", ctypes.c_int),
+                ("N", ctypes.POINTER(ctypes.c_int)), ("a", ctypes.c_double * 3), ("aScale", ctypes.c_double),
+                ("device", ctypes.c_int)]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -78,6 +94,19 @@ def load_library(path=LIB_PATH):
         "tcym_vacuum_ncs": [V, I, _dp], "tcym_timing": [V, I],
         "tcym_kernel_time": [V, ctypes.c_char_p, _dp, LL], "tcym_timing_reset": [V], "tcym_sync": [V],
         "tcym_device_bytes": [V, LL],
+        "tcym_batch_create": [ctypes.POINTER(BatchParams), ctypes.POINTER(V)], "tcym_batch_destroy": [V],
+        "tcym_batch_sync": [V], "tcym_batch_device_bytes": [V, LL],
+        "tcym_batch_set_links": [V, I, I, _dp], "tcym_batch_get_links": [V, I, I, _dp],
+        "tcym_batch_set_efield": [V, I, I, _dp], "tcym_batch_get_efield": [V, I, I, _dp],
+        "tcym_batch_initial_condition": [V, U64, D, D, I, D, ctypes.POINTER(I)],
+        "tcym_batch_set_step": [V, U32], "tcym_batch_get_step": [V, ctypes.POINTER(U32)],
+        "tcym_batch_leapfrog": [V, I, D], "tcym_batch_langevin": [V, I, D, D, D, U64],
+        "tcym_batch_flow": [V, I, I, D], "tcym_batch_evolve": [V, I, I, I, D, D, D, U64, _dp],
+        "tcym_batch_cooling_start": [V, I, D], "tcym_batch_cooling_update": [V, I, D, _dp],
+        "tcym_batch_ncsdot": [V, I, I, _dp], "tcym_batch_bulk": [V, I, I, _dp],
+        "tcym_batch_gauss": [V, I, I, _dp], "tcym_batch_unitarity": [V, I, _dp],
+        "tcym_batch_tune": [V, I, D, I, ctypes.POINTER(I), _dp], "tcym_batch_timing": [V, I],
+        "tcym_batch_kernel_time": [V, ctypes.c_char_p, _dp, LL], "tcym_batch_timing_reset": [V],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -87,6 +116,8 @@ def load_library(path=LIB_PATH):
     L.tcym_last_error.restype = ctypes.c_char_p
     L.tcym_version.argtypes = []
     L.tcym_version.restype = ctypes.c_char_p
+    L.tcym_batch_last_error.argtypes = [V]
+    L.tcym_batch_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -290,3 +321,146 @@ class Solver:
         b = ctypes.c_longlong()
         self._ck(self.L.tcym_device_bytes(self.h, ctypes.byref(b)))
         return b.value
+
+
+class Batch:
+    """Many independent lattices of uneven sizes in one device allocation
+    (include/tcym_batch.h): every pass runs over all of them in one persistent
+    kernel, and every lattice computes bit-for-bit what a Solver of the same
+    size, precision and seed computes.  The reference runs such ensembles as
+    separate MPI ranks with seed + rank (SU2Solver.cpp:161-164)."""
+
+    def __init__(self, nc, sizes, precision=FP32, a=(1.0, 1.0, 1.0), aScale=1.0, device=0):
+        self.L = load_library()
+        sizes = [(n, n, n) if isinstance(n, int) else tuple(n) for n in sizes]
+        self.nc, self.sizes, self.precision = nc, sizes, precision
+        self.count = len(sizes)
+        self.su = 4 if nc == 2 else 18
+        self.ne = 3 if nc == 2 else 8
+        self.V = [n[0] * n[1] * n[2] for n in sizes]
+        flat = (ctypes.c_int * (3 * self.count))(*[k for n in sizes for k in n])
+        self._flat = flat
+        p = BatchParams(nc, precision, self.count, ctypes.cast(flat, ctypes.POINTER(ctypes.c_int)),
+                        (ctypes.c_double * 3)(*a), aScale, device)
+        h = ctypes.c_void_p()
+        r = self.L.tcym_batch_create(ctypes.byref(p), ctypes.byref(h))
+        if r != OK:
+            raise TcymError(r, "tcym_batch_create failed (group %d, %d lattices, device %d)" % (nc, self.count, device))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tcym_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, r):
+        if r != OK:
+            raise TcymError(r, self.L.tcym_batch_last_error(self.h).decode())
+        return r
+
+    # ---- fields ----
+    def set_links(self, e, U, slot=U_HOT):
+        U = _arr(U, 3 * self.V[e] * self.su)
+        self._ck(self.L.tcym_batch_set_links(self.h, slot, e, U.ctypes.data_as(_dp)))
+
+    def get_links(self, e, slot=U_HOT):
+        U = np.empty(3 * self.V[e] * self.su)
+        self._ck(self.L.tcym_batch_get_links(self.h, slot, e, U.ctypes.data_as(_dp)))
+        return U
+
+    def set_efield(self, e, E, slot=E_HOT):
+        E = _arr(E, 3 * self.V[e] * self.ne)
+        self._ck(self.L.tcym_batch_set_efield(self.h, slot, e, E.ctypes.data_as(_dp)))
+
+    def get_efield(self, e, slot=E_HOT):
+        E = np.empty(3 * self.V[e] * self.ne)
+        self._ck(self.L.tcym_batch_get_efield(self.h, slot, e, E.ctypes.data_as(_dp)))
+        return E
+
+    def initial_condition(self, seed0, sigma_u, sigma_e, maxiter=10000, tol=1e-14):
+        """Lattice e: random links/E with seed0 + e, then GaussRestore (FP64)."""
+        it = (ctypes.c_int * self.count)()
+        self._ck(self.L.tcym_batch_initial_condition(self.h, seed0, sigma_u, sigma_e, maxiter, tol, it))
+        return list(it)
+
+    def set_step(self, step):
+        self._ck(self.L.tcym_batch_set_step(self.h, step))
+
+    def get_step(self):
+        s = ctypes.c_uint32()
+        self._ck(self.L.tcym_batch_get_step(self.h, ctypes.byref(s)))
+        return s.value
+
+    # ---- updates ----
+    def leapfrog(self, nsteps, dt):
+        self._ck(self.L.tcym_batch_leapfrog(self.h, nsteps, dt))
+
+    def langevin(self, nsteps, dt, damping, T, seed0):
+        self._ck(self.L.tcym_batch_langevin(self.h, nsteps, dt, damping, T, seed0))
+
+    def flow(self, nsteps, dtau, u=U_HOT):
+        self._ck(self.L.tcym_batch_flow(self.h, u, nsteps, dtau))
+
+    def evolve(self, mode, nseg, steps, dt, damping=0.0, T=0.0, seed0=0):
+        """Graph-replayed segments of `steps` updates + NCsDot; returns NCsDot [count, nseg]."""
+        out = np.empty(self.count * nseg)
+        self._ck(self.L.tcym_batch_evolve(self.h, mode, nseg, steps, dt, damping, T, seed0, out.ctypes.data_as(_dp)))
+        return out.reshape(self.count, nseg)
+
+    def cooling_start(self, nsteps, dtau):
+        self._ck(self.L.tcym_batch_cooling_start(self.h, nsteps, dtau))
+
+    def cooling_update(self, nsteps, dtau):
+        out = np.empty(self.count)
+        self._ck(self.L.tcym_batch_cooling_update(self.h, nsteps, dtau, out.ctypes.data_as(_dp)))
+        return out
+
+    # ---- measurements (one entry per lattice) ----
+    def ncs_dot(self, u=U_HOT, e=E_HOT):
+        out = np.empty(self.count)
+        self._ck(self.L.tcym_batch_ncsdot(self.h, u, e, out.ctypes.data_as(_dp)))
+        return out
+
+    def bulk(self, u=U_HOT, e=E_HOT):
+        out = np.empty(self.count * 6)
+        self._ck(self.L.tcym_batch_bulk(self.h, u, e, out.ctypes.data_as(_dp)))
+        return out.reshape(self.count, 6)
+
+    def gauss(self, u=U_HOT, e=E_HOT):
+        out = np.empty(self.count * 2)
+        self._ck(self.L.tcym_batch_gauss(self.h, u, e, out.ctypes.data_as(_dp)))
+        return out.reshape(self.count, 2)
+
+    def unitarity(self, u=U_HOT):
+        out = np.empty(self.count * 2)
+        self._ck(self.L.tcym_batch_unitarity(self.h, u, out.ctypes.data_as(_dp)))
+        return out.reshape(self.count, 2)
+
+    def tune(self, mode=1, dt=0.05, reps=3):
+        """Autotune the update kernels; returns ((shape, blocks per CU, XCD split), us per step)."""
+        ch = (ctypes.c_int * 3)()
+        us = ctypes.c_double()
+        self._ck(self.L.tcym_batch_tune(self.h, mode, dt, reps, ch, ctypes.byref(us)))
+        return tuple(ch), us.value
+
+    # ---- timing ----
+    def timing(self, enable=True):
+        self._ck(self.L.tcym_batch_timing(self.h, 1 if enable else 0))
+
+    def timing_reset(self):
+        self._ck(self.L.tcym_batch_timing_reset(self.h))
+
+    def kernel_time(self, name):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        self._ck(self.L.tcym_batch_kernel_time(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def sync(self):
+        self._ck(self.L.tcym_batch_sync(self.h))

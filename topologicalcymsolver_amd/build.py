@@ -17,8 +17,8 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libtcym.so")
-SOURCES = ["tcym_api.hip", "tcym_kernels_su2.hip", "tcym_kernels_su3.hip"]
-DEPS = ["tcym_backend.hpp", "tcym_group.hpp", "tcym_kernels.inc"]
+SOURCES = ["tcym_api.hip", "tcym_batch.hip", "tcym_kernels_su2.hip", "tcym_kernels_su3.hip"]
+DEPS = ["tcym_backend.hpp", "tcym_device.hpp", "tcym_group.hpp", "tcym_kernels.inc", "tcym_reduce.hpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -12,6 +12,8 @@
 
 #include "tcym.h"
 #include "tcym_backend.hpp"
+#include "tcym_reduce.hpp"
+#include "tcym_device.hpp"
 
 using tcym::Backend;
 using tcym::Geom;
@@ -20,34 +22,11 @@ using tcym::kBlock;
 namespace {
 
 // final reduction: block q reduces quantity q of nblk partials in a fixed order
-// (per-thread strided Neumaier sums, then a fixed shuffle tree)
 __global__ __launch_bounds__(256) void k_final(const double* __restrict__ part, long nblk, int nq, unsigned is_max,
                                                double* __restrict__ out) {
     const int q = blockIdx.x;
-    const bool mx = (is_max >> q) & 1;
-    double s = mx ? 0.0 : 0.0, c = 0.0;
-    for (long b = threadIdx.x; b < nblk; b += blockDim.x) {
-        const double v = part[b*nq + q];
-        if (mx) s = fmax(s, v);
-        else {
-            const double t = s + v;
-            c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
-            s = t;
-        }
-    }
-    if (!mx) s += c;
-    __shared__ double sm[4];
-    for (int o = 32; o > 0; o >>= 1) {
-        const double w = __shfl_down(s, o, 64);
-        s = mx ? fmax(s, w) : s + w;
-    }
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = sm[0];
-        for (int w = 1; w < 4; w++) r = mx ? fmax(r, sm[w]) : r + sm[w];
-        out[q] = r;
-    }
+    const double r = tcym::final_reduce_block(part, nblk, nq, q, (is_max >> q) & 1);
+    if (threadIdx.x == 0) out[q] = r;
 }
 
 // a field of nr reals of `bytes` each per (site, mu) fits the 32-bit buffer offsets
@@ -55,11 +34,6 @@ bool field_fits(const int* N, int nr, int bytes) {
     const double V = (double)tcym::padded_sites((long)N[0]*N[1]*N[2]);
     return 3.0*V*nr*bytes < 4294967296.0;
 }
-
-struct Pending {
-    std::string name;
-    hipEvent_t a, b;
-};
 
 }  // namespace
 
@@ -71,12 +45,11 @@ void final_reduce(const Launcher& L, const double* part, long nblk, int nq, cons
 }
 }  // namespace tcym
 
-struct tcym_ctx {
+struct tcym_ctx : tcym::DeviceQueue {
     Geom g{};
-    int nc = 0, prec = 0, device = 0;
+    int nc = 0, prec = 0;
     Backend* be = nullptr;     // storage precision
     Backend* be64 = nullptr;   // FP64 (Gauss-law restoration of the FP32 form)
-    hipStream_t stream = nullptr;
     void* U[TCYM_NUM_U] = {};
     void* E[TCYM_NUM_E] = {};
     void* Uscr = nullptr;
@@ -88,13 +61,6 @@ struct tcym_ctx {
     double* U64 = nullptr;
     double* E64 = nullptr;
     double* map = nullptr;
-    long long bytes = 0;
-    std::string err;
-    bool timing = false;
-    std::vector<Pending> pending;
-    std::vector<hipEvent_t> free_events;
-    std::map<std::string, std::pair<double, long long>> stats;
-    hipError_t last = hipSuccess;
 
     // device fields (sites padded to whole tiles) and host AoS arrays
     long link_elems() const { return 3*tcym::padded_sites(g.V)*be->ns; }
@@ -102,46 +68,6 @@ struct tcym_ctx {
     long host_link_elems() const { return 3*g.V*be->ns; }
     long host_field_elems() const { return 3*g.V*be->ne; }
 
-    hipEvent_t ev() {
-        if (!free_events.empty()) { hipEvent_t e = free_events.back(); free_events.pop_back(); return e; }
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        return e;
-    }
-    void launch(const char* name, const tcym::Launch& f) {
-        if (last != hipSuccess) return;
-        if (timing) {
-            hipEvent_t a = ev(), b = ev();
-            hipEventRecord(a, stream);
-            f(stream);
-            hipEventRecord(b, stream);
-            pending.push_back({name, a, b});
-        } else {
-            f(stream);
-        }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) { last = e; err = std::string("kernel ") + name + ": " + hipGetErrorString(e); }
-    }
-    tcym::Launcher L() { return [this](const char* n, const tcym::Launch& f) { launch(n, f); }; }
-
-    int fail(int code, const std::string& m) { err = m; return code; }
-    int hip(hipError_t e, const char* what) {
-        if (e == hipSuccess) return TCYM_OK;
-        last = e;
-        err = std::string(what) + ": " + hipGetErrorString(e);
-        return TCYM_E_HIP;
-    }
-    int status() { return last == hipSuccess ? TCYM_OK : TCYM_E_HIP; }
-    int sync() {
-        if (last != hipSuccess) return TCYM_E_HIP;
-        return hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    }
-    int alloc(void** p, size_t n, const char* what) {
-        if (*p) return TCYM_OK;
-        int r = hip(hipMalloc(p, n), what);
-        if (r == TCYM_OK) { bytes += (long long)n; r = hip(hipMemsetAsync(*p, 0, n, stream), what); }
-        return r;
-    }
     int useU(int slot) {
         if (slot < 0 || slot >= TCYM_NUM_U) return fail(TCYM_E_ARG, "link slot out of range");
         return alloc(&U[slot], (size_t)link_elems()*be->elsize, "hipMalloc(links)");
@@ -236,14 +162,8 @@ int tcym_create(const tcym_params* p, tcym_ctx** out) {
 int tcym_destroy(tcym_ctx* c) {
     if (!c) return TCYM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
-    for (auto& q : c->pending) { hipEventDestroy(q.a); hipEventDestroy(q.b); }
-    for (auto e : c->free_events) hipEventDestroy(e);
-    for (void* p : c->U) if (p) hipFree(p);
-    for (void* p : c->E) if (p) hipFree(p);
-    void* bufs[] = {c->Uscr, c->part, c->red, c->stage, c->G64, c->U64, c->E64, c->map};
-    for (void* p : bufs) if (p) hipFree(p);
     if (c->red_host) hipHostFree(c->red_host);
-    if (c->stream) hipStreamDestroy(c->stream);
+    c->release();
     if (c->be64 != c->be) delete c->be64;
     delete c->be;
     delete c;
@@ -572,36 +492,21 @@ int tcym_vacuum_ncs(tcym_ctx* c, int u, double* out) {
 }
 
 // ---------------------------------------------------------------- timing
-static int resolve(tcym_ctx* c) {
-    int r = c->sync();
-    if (r) return r;
-    for (auto& q : c->pending) {
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, q.a, q.b);
-        auto& s = c->stats[q.name];
-        s.first += ms;
-        s.second += 1;
-        c->free_events.push_back(q.a);
-        c->free_events.push_back(q.b);
-    }
-    c->pending.clear();
-    return TCYM_OK;
-}
 int tcym_timing(tcym_ctx* c, int enable) {
     if (!c) return TCYM_E_ARG;
-    int r = resolve(c);
+    int r = c->resolve();
     c->timing = enable != 0;
     return r;
 }
 int tcym_timing_reset(tcym_ctx* c) {
     if (!c) return TCYM_E_ARG;
-    int r = resolve(c);
+    int r = c->resolve();
     c->stats.clear();
     return r;
 }
 int tcym_kernel_time(tcym_ctx* c, const char* name, double* total_ms, long long* launches) {
     if (!c || !name) return TCYM_E_ARG;
-    int r = resolve(c);
+    int r = c->resolve();
     if (r) return r;
     auto it = c->stats.find(name);
     if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.first;

@@ -18,6 +18,40 @@ struct Geom {
 using Launch = std::function<void(hipStream_t)>;
 using Launcher = std::function<void(const char* name, const Launch&)>;
 
+// geometry as the kernels see it: extents, sites and the walk shape
+struct KGeom {
+    int nx, ny, nz;
+    long V;
+    int pxs, tcx, tcy;   // traversal shape (WalkShape); pxs == 0 is the linear walk
+};
+
+// One lattice of a batch (device table).  Each pass over a batch has its own
+// tiling: `gu`/`utile0` for the update kernels (the autotuner picks the walk
+// shape), `gr`/`ltile0`/`stile0` for the measurement kernels (the shape a
+// single lattice of the same size uses, so that their sums are bit-identical
+// to the single-lattice ones).
+struct BatchEns {
+    KGeom gu, gr;
+    uint64_t u_off, e_off;     // element offsets of the lattice's link / E records
+    uint64_t seed;             // Langevin noise key
+    uint32_t utile0;           // first tile of its update pass (3 per site block)
+    uint32_t ltile0;           // first tile of its measurement pass over links
+    uint32_t stile0;           // first tile of its measurement pass over sites
+    uint32_t pad;
+};
+
+// device tables of a batch handed to the kernels (owned by the batch object)
+struct BatchDev {
+    Geom g;                            // spacings (common to the batch); extents unused
+    const BatchEns* ens;
+    const uint32_t* utiles; uint32_t n_utiles;   // tile -> lattice, update pass
+    const uint32_t* ltiles; uint32_t n_ltiles;   // tile -> lattice, measurement pass over links
+    const uint32_t* stiles; uint32_t n_stiles;   // tile -> lattice, measurement pass over sites
+    int grid_update;                   // persistent grid of the update kernels (blocks)
+    int grid_measure;                  // persistent grid of the measurement kernels
+    const uint32_t* step;              // device step counter (Philox step of step offset 0)
+};
+
 struct Backend {
     int ns = 0, ne = 0, elsize = 0, nc = 0;
     virtual ~Backend() {}
@@ -51,6 +85,17 @@ struct Backend {
     virtual void gauss_correct(const Launcher&, const Geom&, const double* U, double* E, const double* G, double gamma) = 0;
     virtual void geodesic(const Launcher&, const Geom&, double c, const void* Uold, const void* Unew, void* Umid, void* Emid) = 0;
     virtual void reunitarize(const Launcher&, const Geom&, void* U) = 0;
+
+    // batches of lattices (tcym_batch.hip): one persistent launch per pass
+    // over every lattice of the batch, same per-link / per-site arithmetic as
+    // the single-lattice kernels.  Partials go to part[tile*nq + q].
+    virtual void update_batch(const Launcher&, const BatchDev&, int mode, bool write_e, double dt, double damping,
+                              double noise_sd, uint32_t step_off, const void* Uin, void* Uout, void* E, const void* Xi) = 0;
+    virtual void edotb_batch(const Launcher&, const BatchDev&, const void* U, const void* E, double* part) = 0;
+    virtual void bulk_batch(const Launcher&, const BatchDev&, const void* U, const void* E, double* part) = 0;
+    virtual void gauss_batch(const Launcher&, const BatchDev&, const void* U, const void* E, double* part) = 0;
+    virtual void unitarity_batch(const Launcher&, const BatchDev&, const void* U, double* part) = 0;
+    virtual void geodesic_batch(const Launcher&, const BatchDev&, double c, const void* Uo, const void* Un, void* Um, void* Em) = 0;
 };
 
 Backend* make_backend_su2(int fp32);
@@ -64,33 +109,41 @@ constexpr int kBlock = 256;
 // sites of a device field: the tiled layout stores sites in tiles of 64
 inline long padded_sites(long V) { return (V + 63) & ~63L; }
 
-// Traversal of the lattice by the stencil kernels.  A block covers a patch of
-// (1 << pxs) sites in x by (kBlock >> pxs) rows in y of one z-plane; the
-// patches stacked in y make a column kColRows sites high, and a column is
-// swept through all z before the next column.  pxs is 6, 5 or 4, whichever
-// pads Nx least among the patches that fit in a column (ties go to the wider
-// patch, for coalescing).
+// Traversal of the lattice by the stencil kernels.  Patch walk (pxs > 0): a
+// block covers a patch of (1 << pxs) sites in x by (kBlock >> pxs) rows in y
+// of one z-plane; the patches stacked in y make a column kColRows sites high,
+// and a column is swept through all z before the next column.  pxs is 6, 5
+// or 4, whichever pads Nx least among the patches that fit in a column (ties
+// go to the wider patch, for coalescing).  Linear walk (pxs == 0): a block
+// covers kBlock consecutive sites in natural order; it is used when it pads
+// less than every patch shape (small or odd lattices).
 #ifndef TCYM_COL_ROWS
 #define TCYM_COL_ROWS 16
 #endif
 constexpr int kColRows = TCYM_COL_ROWS;
 struct WalkShape {
-    int pxs;           // log2 of the patch width in x
-    int tcx, tcy;      // columns in x and y
+    int pxs;           // log2 of the patch width in x, 0 for the linear walk
+    int tcx, tcy;      // columns in x and y (patch walk)
     long blocks;       // blocks for one pass over the sites
 };
+inline WalkShape patch_shape(const Geom& g, int pxs) {
+    WalkShape w{pxs, 0, 0, 0};
+    w.tcx = (g.n[0] + (1 << pxs) - 1) >> pxs;
+    w.tcy = (g.n[1] + kColRows - 1)/kColRows;
+    w.blocks = (long)w.tcx*w.tcy*g.n[2]*(kColRows/(kBlock >> pxs));
+    return w;
+}
+inline WalkShape linear_shape(const Geom& g) { return WalkShape{0, 0, 0, (g.V + kBlock - 1)/kBlock}; }
+// the shape the measurement kernels use (and the default of the update kernels)
 inline WalkShape walk_shape(const Geom& g) {
-    WalkShape w{6, 0, 0, 0};
-    long best = -1;
+    WalkShape best{-1, 0, 0, 0};
     for (int s = 6; s >= 4; s--) {
         if ((kBlock >> s) > kColRows) continue;          // a patch must fit in a column
-        const long padded = (long)((g.n[0] + (1 << s) - 1) >> s) << s;
-        if (best < 0 || padded < best) { best = padded; w.pxs = s; }
+        const WalkShape w = patch_shape(g, s);
+        if (best.pxs < 0 || w.blocks < best.blocks) best = w;
     }
-    w.tcx = (g.n[0] + (1 << w.pxs) - 1) >> w.pxs;
-    w.tcy = (g.n[1] + kColRows - 1)/kColRows;
-    w.blocks = (long)w.tcx*w.tcy*g.n[2]*(kColRows/(kBlock >> w.pxs));
-    return w;
+    const WalkShape lin = linear_shape(g);
+    return lin.blocks < best.blocks ? lin : best;
 }
 
 }  // namespace tcym
