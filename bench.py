@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -130,6 +132,68 @@ def run_form(T, nc, N, precision, U, E, steps, warmup, dt, cool_every, cool_step
                 stats=stats, unitarity=umax, gauss=gauss)
 
 
+def batched_sizes(count, seed):
+    """Lattice extents L^3, L drawn uniformly from {16, 24, 32, 40, 48} by a
+    Philox generator with the given seed (numpy's Philox-4x64)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    return [int(L) for L in rng.choice([16, 24, 32, 40, 48], size=count)]
+
+
+def run_batched(T, count, seed, precision, lang_steps, seg, cool_every, cool_steps, dt, dtau, damping, temp, tune):
+    """The batching form: `count` independent SU(2) lattices of uneven sizes in
+    one batch (one persistent launch per pass over all of them); Langevin
+    steps in graph-replayed segments of `seg` steps, each ending with NCsDot of
+    every lattice, and a cooled Delta N_CS of every lattice every `cool_every`
+    steps (`cool_steps` flow steps).  Returns the timings and trajectories."""
+    sizes = batched_sizes(count, seed)
+    b = T.Batch(2, sizes, precision)
+    t0 = time.perf_counter()
+    iters = b.initial_condition(seed, 0.2, 0.3, maxiter=10000, tol=1e-14)
+    t_ic = time.perf_counter() - t0
+    choice, best_us = (None, None)
+    if tune:
+        choice, best_us = b.tune(mode=1, dt=dt, reps=3)
+    # warm-up: one segment on the same path (the graph is captured here), then restart the fields
+    links = [b.get_links(e) for e in range(count)]
+    efield = [b.get_efield(e) for e in range(count)]
+    b.evolve(1, 1, seg, dt, damping, temp, seed + 1000)
+    b.cooling_start(2, dtau)
+    for e in range(count):
+        b.set_links(e, links[e])
+        b.set_efield(e, efield[e])
+    b.set_step(0)
+    b.sync()
+    b.timing(True)
+    b.timing_reset()
+    ncsdot, dncs = [], []
+    t0 = time.perf_counter()
+    b.cooling_start(cool_steps, dtau)
+    n_flow = cool_steps
+    done = 0
+    while done < lang_steps:
+        nseg = min(cool_every, lang_steps - done) // seg
+        ncsdot.append(b.evolve(1, nseg, seg, dt, damping, temp, seed + 1000))
+        done += nseg * seg
+        dncs.append(b.cooling_update(cool_steps, dtau))
+        n_flow += cool_steps
+    b.sync()
+    wall = time.perf_counter() - t0
+    kern = {}
+    for name in ("batch_langevin", "batch_flow", "batch_edotb", "batch_geodesic", "batch_reduce", "copy"):
+        ms, n = b.kernel_time(name)
+        if n:
+            kern[name] = {"launches": n, "avg_us": 1e3 * ms / n}
+    b.timing(False)
+    un = b.unitarity()
+    ga = b.gauss()
+    b.close()
+    links_total = sum(3 * L ** 3 for L in sizes)
+    return dict(sizes=sizes, wall=wall, t_ic=t_ic, iters=iters, choice=choice, best_us=best_us,
+                lu=links_total * (lang_steps + n_flow) / wall, links=links_total, n_lang=lang_steps, n_flow=n_flow,
+                ncsdot=np.concatenate(ncsdot, axis=1), dncs=np.cumsum(np.array(dncs).T, axis=1), kernels=kern,
+                unitarity=float(un[:, 0].max()), gauss=float(np.abs(ga[:, 0]).max()))
+
+
 def cpu_baseline(nc, N, U, E, dt, dtau, budget_s):
     """The oracle (FP64 CPU restatement) on the host cores: leapfrog and flow
     steps on the same lattice, until about budget_s seconds are spent."""
@@ -176,6 +240,9 @@ def main():
     ap.add_argument("--restore-maxiter", type=int, default=10000)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 form")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensembles workload")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=500)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -263,6 +330,42 @@ def main():
                        "delta_ncs": r64["ncs"][-1] if r64["ncs"] else None}
         out["fp32_vs_fp64"] = {"speedup": lu32 / lu64, "max_abs_delta_ncs_diff": dn,
                                "max_rel_energy_diff": drift}
+    if not args.no_batched:
+        log("# batched: %d SU(2) lattices, %d Langevin steps ..." % (args.batch_count, args.batch_steps))
+        bkw = dict(lang_steps=args.batch_steps, seg=10, cool_every=100, cool_steps=200, dt=0.05, dtau=0.02,
+                   damping=0.1, temp=0.5)
+        b32 = run_batched(T, args.batch_count, 4004, T.FP32, tune=True, **bkw)
+        log("# batched FP32: %.3f s, %.3e link-updates/s, tuner %s" % (b32["wall"], b32["lu"], b32["choice"]))
+        b64 = run_batched(T, args.batch_count, 4004, T.FP64, tune=True, **bkw)
+        log("# batched FP64: %.3f s, %.3e link-updates/s" % (b64["wall"], b64["lu"]))
+        kd = max(b32["kernels"], key=lambda k: b32["kernels"][k]["avg_us"] * b32["kernels"][k]["launches"])
+        kind = {"batch_langevin": "langevin", "batch_flow": "flow"}.get(kd)
+        roof = None
+        if kind:
+            bpl = bytes_per_link_update(2, kind, True)
+            ach = b32["links"] * bpl / (b32["kernels"][kd]["avg_us"] * 1e-6) / 1e9
+            roof = {"bound": "hbm", "kernel": kd, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": ach / HBM_PEAK_GBS, "bytes_per_link_update": bpl,
+                    "avg_launch_us": b32["kernels"][kd]["avg_us"],
+                    "note": "the batch's hot links and E fields (%.0f MB in FP32) fit in the 256 MB Infinity Cache, "
+                            "so this kernel can run above the HBM roofline" % (b32["links"] * (16 + 12) / 1e6)}
+        out["batched"] = {
+            "workload": "%d independent SU(2) lattices L^3, L in {16,24,32,40,48} (counts %s), FP32-mixed; "
+                        "%d Langevin steps dt=0.05 damping=0.1 T=0.5 in graph-replayed segments of 10 steps "
+                        "ending with NCsDot, 200-step cooling dtau=0.02 every 100 steps (geodesic Delta N_CS)"
+                        % (args.batch_count, {L: b32["sizes"].count(L) for L in (16, 24, 32, 40, 48)}, args.batch_steps),
+            "link_updates_per_s": b32["lu"], "wall_s": b32["wall"], "links": b32["links"],
+            "langevin_steps": b32["n_lang"], "flow_steps": b32["n_flow"],
+            "tuner_choice": {"shape": b32["choice"][0], "blocks_per_cu": b32["choice"][1], "xcd_split": b32["choice"][2],
+                             "best_langevin_us": b32["best_us"]},
+            "kernels": b32["kernels"], "roofline": roof,
+            "fp64_link_updates_per_s": b64["lu"], "fp32_vs_fp64": b32["lu"] / b64["lu"],
+            "max_abs_delta_ncs_diff_vs_fp64": float(np.abs(b32["dncs"] - b64["dncs"]).max()),
+            "max_abs_ncsdot_diff_vs_fp64": float(np.abs(b32["ncsdot"] - b64["ncsdot"]).max()),
+            "delta_ncs_final": {"min": float(b32["dncs"][:, -1].min()), "max": float(b32["dncs"][:, -1].max())},
+            "unitarity_max": b32["unitarity"], "gauss_abs_max": b32["gauss"],
+            "gauss_restore_iterations": {"min": int(min(b32["iters"])), "max": int(max(b32["iters"]))},
+        }
     if args.cpu_seconds > 0:
         log("# CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(nc, N, U, E, args.dt, args.dtau, args.cpu_seconds)
