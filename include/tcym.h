@@ -107,6 +107,16 @@ int tcym_update_links(tcym_ctx *ctx, int u, int e, double dt);
    (DYNAMICS/LangevinDynamics.cpp:260-347; the reference has damping 1, T = 1/beta) */
 int tcym_langevin(tcym_ctx *ctx, int u, int e, int nsteps, double dt, double damping, double T,
                   uint64_t seed, uint32_t step0);
+/* Hot-trajectory N_CS: nsteps of leapfrog (mode 0) or Langevin (mode 1, as
+   tcym_langevin with seed and step0), adding to *dncs after every step
+   dt/2 [NCsDot(U_n, E_n+1/2) + NCsDot(U_n+1, E_n+1/2)].  The reference
+   declares this integral (ChernSimonsNumber::DeltaNCsRealTime,
+   TOPOLOGY/ChernSimonsDerivative.cpp:8) and resets it
+   (SphaleronSimulation.cpp:405) but never accumulates it; the rule mirrors
+   the calibration flow's (GRADIENTFLOW/GradientFlow.cpp:168-188).  The sum
+   stays on the device until the end of the call. */
+int tcym_evolve_ncs(tcym_ctx *ctx, int mode, int u, int e, int nsteps, double dt, double damping, double T,
+                    uint64_t seed, uint32_t step0, double *dncs);
 /* LangevinDynamics::UpdateFields with the noise slot xi (DYNAMICS/LangevinDynamics.cpp:300-311) */
 int tcym_langevin_xi(tcym_ctx *ctx, int u, int e, int xi, double dt, double damping);
 /* LangevinDynamics::UpdateRandomGaussianNoise into slot xi (DYNAMICS/LangevinDynamics.cpp:260-286) */
@@ -124,6 +134,36 @@ int tcym_cool(tcym_ctx *ctx, int u_hot, int u_cold, int nsteps, double dtau);
    TCYM_U_OLD and TCYM_U_NEW; then TCYM_U_OLD <- TCYM_U_NEW */
 int tcym_cooling_start(tcym_ctx *ctx, int u_hot, int nsteps, double dtau);
 int tcym_cooling_update(tcym_ctx *ctx, int u_hot, int nsteps, double dtau, int measure, double *dncs);
+
+/* ---- blocking and calibration ---- */
+/* Block(GaugeLinks**) / Block(ElectricFields**) (LATTICE/GaugeLinks.cpp:125-151,
+   LATTICE/ElectricFields.cpp:113-127): the links of slot u become the lattice
+   of half the extent and twice the spacing, U'_mu(x) = U_mu(2x) U_mu(2x+mu);
+   E slot e (if >= 0) becomes zero at the new resolution.  A slot keeps its
+   resolution level (0 = the full lattice) until it is overwritten; every entry
+   point works at the level of the slots it is given (mixing levels is an
+   error), and a level-l slot reads and writes host arrays of the level-l size. */
+int tcym_block(tcym_ctx *ctx, int u, int e);
+/* resolution of link slot u or (u < 0) field slot e: blocking level and extents */
+int tcym_level(tcym_ctx *ctx, int u, int e, int *level, int N[3]);
+/* Cooling::CoolNSave with blocking (GRADIENTFLOW/Cooling.cpp:69-179): u_cold := u_hot,
+   then nsteps fused flow steps; after step k (from 1) the cold links are blocked
+   when k % (block_frequency 8^b) == 0 and b < max_blocking, b the blockings done
+   so far (block_frequency 0: never).  *nblocked (may be NULL) receives b. */
+int tcym_cool_blocked(tcym_ctx *ctx, int u_hot, int u_cold, int nsteps, double dtau, int block_frequency,
+                      int max_blocking, int *nblocked);
+/* Cooling::ContinueCooling with GradientFlow::CalibrationUpdate (GRADIENTFLOW/Cooling.cpp:181-195,
+   69-147; GradientFlow.cpp:168-188): u_work := u_src with E slot e_work zero, then the
+   calibration flow steps step0+1 .. nsteps (the reference's CoolingSteps continue from
+   the preceding cooling), blocking as in tcym_cool_blocked with the blocking count
+   continuing from *nblocked (in/out).  *dncs (may be NULL) receives DeltaNCsCooling
+   along these steps; log (may be NULL) receives nsteps - step0 + 1 rows of
+   {CoolingTime, Bulk::MAGNETIC, Bulk::ELECTRIC, DeltaNCsCooling, VacuumEstimator::NCS}
+   before the first step and after each step (Cooling.cpp:85-111), the cooling time
+   starting at ctau0 (GradientFlow::cTau). */
+int tcym_continue_cooling(tcym_ctx *ctx, int u_src, int u_work, int e_work, int step0, int nsteps, double dtau,
+                          double ctau0, int block_frequency, int max_blocking, int *nblocked, double *dncs,
+                          double *log);
 /* FP64 reunitarisation of every link (no reference counterpart: the FP32 form applies it after each step) */
 int tcym_reunitarize(tcym_ctx *ctx, int u);
 
@@ -137,6 +177,10 @@ int tcym_ncsdot(tcym_ctx *ctx, int u, int e, double *out);
 int tcym_edotb_map(tcym_ctx *ctx, int u, int e, double *map);
 /* Observables::Bulk::Update: out = {ESqr[3], BSqr[3]} (OBSERVABLES/BulkObservables.cpp:133-143) */
 int tcym_bulk(tcym_ctx *ctx, int u, int e, double out[6]);
+/* Observables::HardScales::Update: H[mu] = sum_x sum_a (2 a^3 F_mu^a)^2 / V,
+   F the force of the E-field update (OBSERVABLES/HardScales.cpp:40-104);
+   LambdaXX/YY/ZZ are combinations of H on the host (:26-36) */
+int tcym_hard_scales(tcym_ctx *ctx, int u, double out[3]);
 /* Observables::GaussLaw::UpdateViolation: signed max and RMS (OBSERVABLES/GaussLawViolation.cpp:13-61) */
 int tcym_gauss(tcym_ctx *ctx, int u, int e, double *signed_max, double *rms);
 /* Observables::Unitarity::UpdateViolation (OBSERVABLES/UnitarityViolation.cpp:13-51) */

@@ -385,7 +385,34 @@ inline void Update(GaugeLinks* U, ElectricFields* E) {   // OBSERVABLES/BulkObse
 inline DOUBLE T00() { return 0.5*(cE()[0]*ESqr()[0] + cE()[1]*ESqr()[1] + cE()[2]*ESqr()[2] + cB()[0]*BSqr()[0] + cB()[1]*BSqr()[1] + cB()[2]*BSqr()[2]); }
 inline DOUBLE ELECTRIC() { return 0.5*(cE()[0]*ESqr()[0] + cE()[1]*ESqr()[1] + cE()[2]*ESqr()[2]); }
 inline DOUBLE MAGNETIC() { return 0.5*(cB()[0]*BSqr()[0] + cB()[1]*BSqr()[1] + cB()[2]*BSqr()[2]); }
+// diagonal pressure components T_ii, in the reference's order of terms
+// (OBSERVABLES/BulkObservables.cpp:33-44)
+inline DOUBLE Tii(int i) {
+    const int j = (i + 1) % 3, k = (i + 2) % 3;
+    const int lo = j < k ? j : k, hi = j < k ? k : j;
+    return 0.5*((cE()[lo]*ESqr()[lo] + cE()[hi]*ESqr()[hi] + cB()[lo]*BSqr()[lo] + cB()[hi]*BSqr()[hi])
+                - (cE()[i]*ESqr()[i] + cB()[i]*BSqr()[i]));
+}
+inline DOUBLE TXX() { return Tii(0); }
+inline DOUBLE TYY() { return Tii(1); }
+inline DOUBLE TZZ() { return Tii(2); }
 }  // namespace Bulk
+namespace HardScales {
+inline DOUBLE* H() { static DOUBLE v[3]; return v; }
+inline DOUBLE* cE() { static DOUBLE v[3]; return v; }
+inline void Update(GaugeLinks* U) {   // OBSERVABLES/HardScales.cpp:40-110
+    auto& d = tcym_dropin::dev(U);
+    d.check(tcym_hard_scales(d.ctx, d.links(U, TCYM_U_AUX), H()));
+    for (int mu = 0; mu < 3; mu++) {
+        const double c = U->a[mu]*Lattice::aScale()*Lattice::aScale()/U->aCube;   // :16-22 (Minkowski)
+        cE()[mu] = c*c;
+    }
+}
+// characteristic momentum scales (OBSERVABLES/HardScales.cpp:26-36)
+inline DOUBLE LambdaXX() { return cE()[1]*H()[1] + cE()[2]*H()[2] - cE()[0]*H()[0]; }
+inline DOUBLE LambdaYY() { return cE()[2]*H()[2] + cE()[0]*H()[0] - cE()[1]*H()[1]; }
+inline DOUBLE LambdaZZ() { return cE()[0]*H()[0] + cE()[1]*H()[1] - cE()[2]*H()[2]; }
+}  // namespace HardScales
 namespace GaussLaw {
 inline DOUBLE& GlobalMaxViolation() { static DOUBLE v; return v; }
 inline DOUBLE& GlobalAvgViolation() { static DOUBLE v; return v; }

@@ -194,6 +194,12 @@ double or_flow_calibration(const or_params *p, double *U, double *E, int n, doub
     }
     return d;
 }
+void or_block_links(const or_params *p, const double *U, double *Uc) {
+    DISPATCH(p, su2l_block_links(p, U, Uc), su3l_block_links(p, U, Uc));
+}
+void or_hard_scales(const or_params *p, const double *U, double out[3]) {
+    DISPATCH(p, su2l_hard_scales(p, U, out), su3l_hard_scales(p, U, out));
+}
 void or_bulk(const or_params *p, const double *U, const double *E, double out[6]) {
     DISPATCH(p, su2l_bulk(p, U, E, out), su3l_bulk(p, U, E, out));
 }

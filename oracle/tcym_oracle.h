@@ -75,6 +75,10 @@ double or_edotb_unimproved(const or_params *p, const double *U, const double *E)
 void   or_edotb_map(const or_params *p, const double *U, const double *E, double *map);
 double or_ncsdot(const or_params *p, const double *U, const double *E);
 void   or_bulk(const or_params *p, const double *U, const double *E, double out[6]);
+/* Block(GaugeLinks**): coarse links U_mu(2x) U_mu(2x+mu) on N/2, 2a (LATTICE/GaugeLinks.cpp:125-151) */
+void   or_block_links(const or_params *p, const double *U, double *Uc);
+/* Observables::HardScales H[3] (OBSERVABLES/HardScales.cpp:40-104) */
+void   or_hard_scales(const or_params *p, const double *U, double out[3]);
 void   or_gauss(const or_params *p, const double *U, const double *E, double *smax, double *rms);
 void   or_gauss_field(const or_params *p, const double *U, const double *E, double *G);
 void   or_unitarity(const or_params *p, const double *U, double *umax, double *rms);

@@ -59,6 +59,8 @@ def lib():
             "or_edotb": (d, [P, _dp, _dp]), "or_edotb_unimproved": (d, [P, _dp, _dp]),
             "or_edotb_map": (None, [P, _dp, _dp, _dp]), "or_ncsdot": (d, [P, _dp, _dp]),
             "or_bulk": (None, [P, _dp, _dp, _dp]),
+            "or_hard_scales": (None, [P, _dp, _dp]),
+            "or_block_links": (None, [P, _dp, _dp]),
             "or_gauss": (None, [P, _dp, _dp, _dp, _dp]), "or_gauss_field": (None, [P, _dp, _dp, _dp]),
             "or_unitarity": (None, [P, _dp, _dp, _dp]),
             "or_gauss_restore": (i, [P, _dp, _dp, i, d, _dp]),
@@ -164,6 +166,19 @@ class Lattice:
     def bulk(self):
         o = np.zeros(6)
         lib().or_bulk(ctypes.byref(self.p), _ptr(self.U), _ptr(self.E), _ptr(o))
+        return o
+
+    def blocked(self):
+        """Block(GaugeLinks**): a new Lattice of half the extent and twice the spacing
+        with the blocked links (E zero, as Block(ElectricFields**))."""
+        a = tuple(2.0 * x for x in self.p.a)
+        c = Lattice(self.nc, tuple(n // 2 for n in self.N), a=a, aScale=self.p.aScale, nthreads=self.p.nthreads)
+        lib().or_block_links(ctypes.byref(self.p), _ptr(self.U), _ptr(c.U))
+        return c
+
+    def hard_scales(self):
+        o = np.zeros(3)
+        lib().or_hard_scales(ctypes.byref(self.p), _ptr(self.U), _ptr(o))
         return o
 
     def energy(self):

@@ -38,8 +38,11 @@ int main(int argc, char** argv) {
         Observables::Bulk::Update(&U, &E);
         Observables::GaussLaw::UpdateViolation(0, N - 1, 0, N - 1, 0, N - 1, &U, &E);
         Observables::Unitarity::UpdateViolation(0, N - 1, 0, N - 1, 0, N - 1, &U);
-        printf("%.17g %.17g %.17g %.17g\n", ncs, Observables::Bulk::T00(),
-               Observables::GaussLaw::GlobalMaxViolation(), Observables::Unitarity::GlobalMaxViolation());
+        Observables::HardScales::Update(&U);
+        printf("%.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", ncs, Observables::Bulk::T00(),
+               Observables::GaussLaw::GlobalMaxViolation(), Observables::Unitarity::GlobalMaxViolation(),
+               Observables::Bulk::TXX(), Observables::Bulk::TZZ(), Observables::HardScales::LambdaXX(),
+               Observables::HardScales::LambdaZZ());
         wr(std::string(argv[5]) + "_U.bin", U.U, nu);
         wr(std::string(argv[5]) + "_E.bin", E.E, ne);
         {

@@ -238,3 +238,131 @@ def test_trajectory_su2_16_matches_oracle():
     assert np.abs(np.array(ncs_gpu) - np.array(ncs_or)).max() <= 1e-10 * scale, (ncs_gpu[-1], ncs_or[-1])
     assert abs(gpu.gauss_violation()[1] - lat.gauss()[1]) < 1e-12
     assert abs(gpu.unitarity_violation()[0] - lat.unitarity()[0]) < 1e-12
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_hot_trajectory_ncs_on_device(nc, N, mode):
+    """tcym_evolve_ncs keeps the hot-trajectory integral on the device; it must
+    equal the same sum composed on the host from per-step NCsDot calls (the
+    composition that test_trajectory_su2_16_matches_oracle pins to the oracle)."""
+    lat, gpu = pair(nc, N, 21, 0.5, 0.4, restore=True)
+    ref = T.Solver(nc, N)
+    ref.set_links(lat.U)
+    ref.set_efield(lat.E)
+    dt, steps = 0.05, 12
+    d_dev = gpu.evolve_ncs(mode, steps, dt, 0.3, 0.5, 77, 5)
+    d_host = 0.0
+    for k in range(steps):
+        ref.copy_links(T.U_AUX, T.U_HOT)
+        if mode == 1:
+            ref.langevin_update(1, dt, 0.3, 0.5, 77, 5 + k)
+        else:
+            ref.dynamics_update(1, dt)
+        d_host += 0.5 * dt * (ref.ncs_dot(u=T.U_AUX) + ref.ncs_dot())
+    assert d_dev == d_host
+    assert np.array_equal(gpu.get_links(), ref.get_links())
+    assert np.array_equal(gpu.get_efield(), ref.get_efield())
+    # leapfrog against the oracle as well
+    if mode == 0:
+        n_or = 0.0
+        for k in range(steps):
+            U0 = lat.U.copy()
+            lat.leapfrog(1, dt)
+            n_or += 0.5 * dt * (lat.ncsdot(U0, lat.E) + lat.ncsdot())
+        assert abs(d_dev - n_or) <= 1e-10 * max(abs(n_or), 1e-12)
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_fp32_langevin_noise(nc, N):
+    """FP32 form: the in-kernel Langevin noise (Box-Muller in FP32 from the
+    same 53-bit Philox uniforms) equals the noise-field path bit for bit and
+    the oracle's FP64 draw to FP32 rounding."""
+    lat, gpu = pair(nc, N, 10, 0.5, 0.3, precision=T.FP32)
+    sd = np.sqrt(2 * 1.0 * 0.5)
+    gpu.copy_links(T.U_AUX, T.U_HOT)
+    gpu.copy_efield(T.E_AUX, T.E_HOT)
+    gpu.langevin_update(1, 0.05, 1.0, 0.5, seed=5, step0=11)
+    gpu.random_gaussian_noise(5, 11, sd)
+    Xi = lat.noise(5, 11, sd)
+    xi32 = gpu.get_efield(T.E_XI)
+    np.testing.assert_allclose(xi32, Xi, rtol=5e-6, atol=5e-6 * sd)
+    gpu.langevin_update_fields(0.05, 1.0, u=T.U_AUX, e=T.E_AUX)
+    assert np.array_equal(gpu.get_links(T.U_AUX), gpu.get_links())
+    assert np.array_equal(gpu.get_efield(T.E_AUX), gpu.get_efield())
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_hard_scales_match_oracle(nc, N):
+    lat, gpu = pair(nc, N, 14, 0.7, 0.6)
+    np.testing.assert_allclose(gpu.hard_scales(), lat.hard_scales(), rtol=1e-12, atol=1e-14)
+
+
+BLOCK_CASES = [(2, (8, 6, 4)), (3, (4, 8, 4))]
+
+
+@pytest.mark.parametrize("nc,N", BLOCK_CASES)
+def test_blocking_matches_oracle(nc, N):
+    lat, gpu = pair(nc, N, 31, 0.6, 0.4)
+    gpu.block(T.U_HOT, T.E_HOT)
+    coarse = lat.blocked()
+    assert gpu.level(u=T.U_HOT) == (1, coarse.N) and gpu.level(e=T.E_HOT) == (1, coarse.N)
+    np.testing.assert_allclose(gpu.get_links(), coarse.U, atol=1e-13)
+    assert not gpu.get_efield().any()
+    # measurements at the coarse resolution (spacing 2)
+    np.testing.assert_allclose(gpu.bulk_update(), coarse.bulk(), rtol=1e-12, atol=1e-14)
+    assert abs(gpu.vacuum_ncs() - coarse.vacuum_ncs()) < 1e-13
+    # a slot at another resolution cannot be mixed in
+    gpu.set_efield(lat.E, T.E_AUX)
+    with pytest.raises(T.TcymError):
+        gpu.bulk_update(u=T.U_HOT, e=T.E_AUX)
+
+
+@pytest.mark.parametrize("nc,N", BLOCK_CASES)
+def test_cool_blocked_matches_oracle(nc, N):
+    """Cooling::CoolNSave with BlockFrequency 3 and MaxBlockingNumber 1: the
+    cold links are blocked after flow step 3 and flowed on at spacing 2."""
+    lat, gpu = pair(nc, N, 32, 0.6, 0.0)
+    nb = gpu.cool_blocked(8, 0.02, 3, 1, u_hot=T.U_HOT, u_cold=T.U_NEW)
+    cold, nbo = lat.copy(), 0
+    for k in range(1, 9):
+        cold.flow(1, 0.02)
+        if k % (3 * 8 ** nbo) == 0 and nbo < 1:
+            cold = cold.blocked()
+            nbo += 1
+    assert nb == nbo == 1
+    np.testing.assert_allclose(gpu.get_links(T.U_NEW), cold.U, atol=1e-12)
+    # hot links untouched, at full resolution
+    np.testing.assert_allclose(gpu.get_links(T.U_HOT), lat.U, atol=0)
+
+
+@pytest.mark.parametrize("nc,N", BLOCK_CASES)
+def test_continue_cooling_matches_oracle(nc, N):
+    """Cooling::ContinueCooling in calibration mode: Delta N_CS along the flow,
+    the per-step log {t, MAGNETIC, ELECTRIC, DeltaNCs, VacEst} and blocking."""
+    lat, gpu = pair(nc, N, 33, 0.6, 0.0)
+    step0, nsteps, dtau, ctau0 = 2, 7, 0.05, 0.3
+    d, nb, log = gpu.continue_cooling(step0, nsteps, dtau, ctau0, block_frequency=4, max_blocking=1,
+                                      u_src=T.U_HOT, u_work=T.U_COOL, e_work=T.E_COOL)
+    work = lat.copy()
+    work.E[:] = 0.0
+    acc, ctau, nbo, rows = 0.0, ctau0, 0, []
+
+    def row(w):
+        b = w.bulk()
+        a = w.p.a[0]
+        cb = (a / a ** 3) ** 2
+        return [ctau, 0.5 * cb * b[3:].sum(), 0.5 * cb * b[:3].sum(), acc, w.vacuum_ncs()]
+
+    rows.append(row(work))
+    for k in range(step0 + 1, nsteps + 1):
+        acc += work.flow_calibration(1, dtau)
+        ctau += dtau * (work.p.a[0] ** 3) ** (2.0 / 3.0)
+        rows.append(row(work))
+        if k % (4 * 8 ** nbo) == 0 and nbo < 1:
+            work = work.blocked()
+            nbo += 1
+    assert nb == nbo == 1
+    assert abs(d - acc) <= 1e-10 * max(abs(acc), 1e-12)
+    np.testing.assert_allclose(log, np.array(rows), rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(gpu.get_links(T.U_COOL), work.U, atol=1e-12)

@@ -287,3 +287,61 @@ def test_reunitarize(nc):
     lat.reunitarize()
     assert lat.unitarity()[0] < 1e-14
     assert np.abs(lat.U - W).max() < 1e-2
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+@pytest.mark.parametrize("a", [(1.0, 1.0, 1.0), (1.0, 1.25, 0.8)])
+def test_hard_scales_from_leapfrog_increment(nc, a):
+    """HardScales H_mu = sum_x sum_a (2 a^3 F_mu^a)^2 / V (OBSERVABLES/HardScales.cpp:40-104).
+    One leapfrog step adds -2 dt a^3 F to E with the old links, so H_mu is
+    also sum (dE/dt)^2 / V of the E-field increment of the leapfrog sweep."""
+    lat = O.Lattice(nc, (4, 5, 3), a=a)
+    lat.random(41, 0.6, 0.4, restore=False)
+    H = lat.hard_scales()
+    E0 = lat.E.copy()
+    dt = 0.01
+    lat.leapfrog(1, dt)
+    dE = ((lat.E - E0) / dt).reshape(lat.V, 3, -1)
+    want = (dE ** 2).sum(axis=(0, 2)) / lat.V
+    np.testing.assert_allclose(H, want, rtol=1e-9)
+    assert (H > 0).all()
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_blocking_is_gauge_covariant(nc):
+    """Block(GaugeLinks**) (LATTICE/GaugeLinks.cpp:125-151): blocking a gauge
+    transform of U equals the transform of the blocked links by G(2x)."""
+    lat = O.Lattice(nc, (6, 4, 8))
+    lat.random(51, 0.7, 0.0, restore=False)
+    G = random_gauge(nc, lat.V, 52)
+    b1 = gauge_transform(lat, G).blocked()
+    b0 = lat.blocked()
+    N = lat.N
+    Gc = G.reshape(N[2], N[1], N[0], -1)[::2, ::2, ::2].reshape(-1)
+    b2 = gauge_transform(b0, Gc)
+    np.testing.assert_allclose(b1.U, b2.U, atol=1e-13)
+    assert b0.N == (3, 2, 4) and tuple(b0.p.a) == (2.0, 2.0, 2.0)
+
+
+def test_blocking_adds_abelian_phases():
+    """For links exp(i theta_mu(x) sigma_3/2) the blocked phase is
+    theta_mu(2x) + theta_mu(2x + mu)."""
+    N = (4, 6, 2)
+    lat = O.Lattice(2, N)
+    r = np.random.default_rng(53)
+    th = r.uniform(-1.0, 1.0, size=(N[2], N[1], N[0], 3))
+    U = lat.U.reshape(N[2], N[1], N[0], 3, 4)
+    U[..., 0] = 0.0
+    U[..., 1] = 0.0
+    U[..., 2] = np.sin(th / 2)
+    U[..., 3] = np.cos(th / 2)
+    b = lat.blocked()
+    Ub = b.U.reshape(N[2] // 2, N[1] // 2, N[0] // 2, 3, 4)
+    for z in range(N[2] // 2):
+        for y in range(N[1] // 2):
+            for x in range(N[0] // 2):
+                f = (2 * z, 2 * y, 2 * x)
+                nb = [(2 * z, 2 * y, 2 * x + 1), (2 * z, 2 * y + 1, 2 * x), (2 * z + 1, 2 * y, 2 * x)]
+                for mu in range(3):
+                    t = th[f + (mu,)] + th[nb[mu] + (mu,)]
+                    np.testing.assert_allclose(Ub[z, y, x, mu], [0, 0, np.sin(t / 2), np.cos(t / 2)], atol=1e-14)

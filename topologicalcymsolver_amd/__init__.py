@@ -27,10 +27,11 @@ EXPORTS = [
     "tcym_set_links", "tcym_get_links", "tcym_set_efield", "tcym_get_efield",
     "tcym_copy_links", "tcym_copy_efield", "tcym_zero_efield", "tcym_unit_links",
     "tcym_random_links", "tcym_random_efield",
-    "tcym_leapfrog", "tcym_update_efield", "tcym_update_links", "tcym_langevin", "tcym_langevin_xi",
+    "tcym_leapfrog", "tcym_update_efield", "tcym_update_links", "tcym_langevin", "tcym_langevin_xi", "tcym_evolve_ncs",
     "tcym_noise", "tcym_flow", "tcym_cool", "tcym_cooling_start", "tcym_cooling_update", "tcym_reunitarize",
-    "tcym_edotb", "tcym_ncsdot", "tcym_edotb_map", "tcym_bulk", "tcym_gauss", "tcym_unitarity",
+    "tcym_edotb", "tcym_ncsdot", "tcym_edotb_map", "tcym_bulk", "tcym_hard_scales", "tcym_gauss", "tcym_unitarity",
     "tcym_gauss_restore", "tcym_geodesic", "tcym_delta_ncs_cooling", "tcym_vacuum_ncs",
+    "tcym_block", "tcym_level", "tcym_cool_blocked", "tcym_continue_cooling",
     "tcym_timing", "tcym_kernel_time", "tcym_timing_reset", "tcym_sync", "tcym_device_bytes",
 ]
 # every exported symbol of include/tcym_batch.h
@@ -84,14 +85,19 @@ def load_library(path=LIB_PATH):
         "tcym_unit_links": [V, I], "tcym_random_links": [V, I, U64, D], "tcym_random_efield": [V, I, U64, D],
         "tcym_leapfrog": [V, I, I, I, D], "tcym_update_efield": [V, I, I, D], "tcym_update_links": [V, I, I, D],
         "tcym_langevin": [V, I, I, I, D, D, D, U64, U32], "tcym_langevin_xi": [V, I, I, I, D, D],
+        "tcym_evolve_ncs": [V, I, I, I, I, D, D, D, U64, U32, _dp],
         "tcym_noise": [V, I, U64, U32, D], "tcym_flow": [V, I, I, I, D, I, _dp],
         "tcym_cool": [V, I, I, I, D], "tcym_cooling_start": [V, I, I, D],
         "tcym_cooling_update": [V, I, I, D, I, _dp], "tcym_reunitarize": [V, I],
         "tcym_edotb": [V, I, I, I, _dp], "tcym_ncsdot": [V, I, I, _dp], "tcym_edotb_map": [V, I, I, _dp],
         "tcym_bulk": [V, I, I, _dp], "tcym_gauss": [V, I, I, _dp, _dp], "tcym_unitarity": [V, I, _dp, _dp],
+        "tcym_hard_scales": [V, I, _dp],
         "tcym_gauss_restore": [V, I, I, I, D, ctypes.POINTER(I), _dp],
         "tcym_geodesic": [V, I, I, I, I, D], "tcym_delta_ncs_cooling": [V, I, I, _dp],
         "tcym_vacuum_ncs": [V, I, _dp], "tcym_timing": [V, I],
+        "tcym_block": [V, I, I], "tcym_level": [V, I, I, ctypes.POINTER(I), ctypes.POINTER(I)],
+        "tcym_cool_blocked": [V, I, I, I, D, I, I, ctypes.POINTER(I)],
+        "tcym_continue_cooling": [V, I, I, I, I, I, D, D, I, I, ctypes.POINTER(I), _dp, _dp],
         "tcym_kernel_time": [V, ctypes.c_char_p, _dp, LL], "tcym_timing_reset": [V], "tcym_sync": [V],
         "tcym_device_bytes": [V, LL],
         "tcym_batch_create": [ctypes.POINTER(BatchParams), ctypes.POINTER(V)], "tcym_batch_destroy": [V],
@@ -169,7 +175,8 @@ class Solver:
         self._ck(self.L.tcym_set_links(self.h, slot, U.ctypes.data_as(_dp)))
 
     def get_links(self, slot=U_HOT):
-        U = np.empty(3 * self.V * self.su)
+        n = self.level(u=slot)[1]
+        U = np.empty(3 * n[0] * n[1] * n[2] * self.su)
         self._ck(self.L.tcym_get_links(self.h, slot, U.ctypes.data_as(_dp)))
         return U
 
@@ -178,7 +185,8 @@ class Solver:
         self._ck(self.L.tcym_set_efield(self.h, slot, E.ctypes.data_as(_dp)))
 
     def get_efield(self, slot=E_HOT):
-        E = np.empty(3 * self.V * self.ne)
+        n = self.level(e=slot)[1]
+        E = np.empty(3 * n[0] * n[1] * n[2] * self.ne)
         self._ck(self.L.tcym_get_efield(self.h, slot, E.ctypes.data_as(_dp)))
         return E
 
@@ -215,6 +223,14 @@ class Solver:
         """LangevinDynamics::Update (DYNAMICS/LangevinDynamics.cpp:320-347)."""
         self._ck(self.L.tcym_langevin(self.h, u, e, nsteps, dt, damping, T, seed, step0))
 
+    def evolve_ncs(self, mode, nsteps, dt, damping=0.0, T=0.0, seed=0, step0=0, u=U_HOT, e=E_HOT):
+        """nsteps leapfrog (mode 0) or Langevin (mode 1) steps; returns the hot-trajectory
+        N_CS integral sum dt/2 [NCsDot(U_n, E_n+1/2) + NCsDot(U_n+1, E_n+1/2)]
+        (ChernSimonsNumber::DeltaNCsRealTime, TOPOLOGY/ChernSimonsDerivative.cpp:8)."""
+        d = ctypes.c_double(0.0)
+        self._ck(self.L.tcym_evolve_ncs(self.h, mode, u, e, nsteps, dt, damping, T, seed, step0, ctypes.byref(d)))
+        return d.value
+
     def langevin_update_fields(self, dt=0.001, damping=1.0, xi=E_XI, u=U_HOT, e=E_HOT):
         """LangevinDynamics::UpdateFields with LangevinVariables::Xi (DYNAMICS/LangevinDynamics.cpp:300-311)."""
         self._ck(self.L.tcym_langevin_xi(self.h, u, e, xi, dt, damping))
@@ -241,6 +257,36 @@ class Solver:
         self._ck(self.L.tcym_cooling_update(self.h, u_hot, nsteps, dtau, 1 if measure else 0, ctypes.byref(d)))
         return d.value
 
+    def block(self, u, e=-1):
+        """Block(GaugeLinks**) / Block(ElectricFields**) (LATTICE/GaugeLinks.cpp:125-151)."""
+        self._ck(self.L.tcym_block(self.h, u, e))
+
+    def level(self, u=-1, e=-1):
+        """(blocking level, extents) of link slot u or field slot e."""
+        lv = ctypes.c_int()
+        n = (ctypes.c_int * 3)()
+        self._ck(self.L.tcym_level(self.h, u, e, ctypes.byref(lv), n))
+        return lv.value, tuple(n)
+
+    def cool_blocked(self, nsteps, dtau, block_frequency, max_blocking, u_hot=U_HOT, u_cold=U_NEW):
+        """Cooling::CoolNSave with blocking (GRADIENTFLOW/Cooling.cpp:69-179); returns the blockings done."""
+        nb = ctypes.c_int()
+        self._ck(self.L.tcym_cool_blocked(self.h, u_hot, u_cold, nsteps, dtau, block_frequency, max_blocking,
+                                          ctypes.byref(nb)))
+        return nb.value
+
+    def continue_cooling(self, step0, nsteps, dtau, ctau0=0.0, block_frequency=0, max_blocking=0, nblocked=0,
+                         u_src=U_NEW, u_work=U_COOL, e_work=E_COOL):
+        """Cooling::ContinueCooling in calibration mode (GRADIENTFLOW/Cooling.cpp:181-195);
+        returns (DeltaNCsCooling, blockings, log rows [t, MAGNETIC, ELECTRIC, DeltaNCs, VacEst])."""
+        nb = ctypes.c_int(nblocked)
+        d = ctypes.c_double()
+        log = np.empty((nsteps - step0 + 1, 5))
+        self._ck(self.L.tcym_continue_cooling(self.h, u_src, u_work, e_work, step0, nsteps, dtau, ctau0,
+                                              block_frequency, max_blocking, ctypes.byref(nb), ctypes.byref(d),
+                                              log.ctypes.data_as(_dp)))
+        return d.value, nb.value, log
+
     def reunitarize(self, u=U_HOT):
         self._ck(self.L.tcym_reunitarize(self.h, u))
 
@@ -265,6 +311,12 @@ class Solver:
         """Observables::Bulk::Update: {ESqr[3], BSqr[3]} (OBSERVABLES/BulkObservables.cpp:133-143)."""
         o = np.empty(6)
         self._ck(self.L.tcym_bulk(self.h, u, e, o.ctypes.data_as(_dp)))
+        return o
+
+    def hard_scales(self, u=U_HOT):
+        """Observables::HardScales::Update: H[mu] (OBSERVABLES/HardScales.cpp:40-104)."""
+        o = np.empty(3)
+        self._ck(self.L.tcym_hard_scales(self.h, u, o.ctypes.data_as(_dp)))
         return o
 
     def energy(self, u=U_HOT, e=E_HOT):

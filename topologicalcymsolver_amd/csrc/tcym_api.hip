@@ -35,6 +35,16 @@ bool field_fits(const int* N, int nr, int bytes) {
     return 3.0*V*nr*bytes < 4294967296.0;
 }
 
+// hot-trajectory N_CS: acc += half_dt (NCsDot_a + NCsDot_b), NCsDot = E.B / (8 pi^2)
+// (no fused multiply-add: the product is rounded before the sum, as in the host composition)
+__global__ void k_hot_ncs(const double* __restrict__ eb, double* __restrict__ acc, double half_dt) {
+#pragma clang fp contract(off)
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double a = 0.125*eb[0]/(M_PI*M_PI), b = 0.125*eb[1]/(M_PI*M_PI);
+        acc[0] += half_dt*(a + b);
+    }
+}
+
 }  // namespace
 
 namespace tcym {
@@ -62,11 +72,37 @@ struct tcym_ctx : tcym::DeviceQueue {
     double* E64 = nullptr;
     double* map = nullptr;
 
+    // Resolution level of each slot: 0 is the full lattice; level l is the
+    // lattice blocked l times (extent N >> l, spacing a << l), which a
+    // blocked cooled copy holds (Block, LATTICE/GaugeLinks.cpp:125-151).
+    // A level-l field occupies the start of its slot's buffer.
+    int ulev[TCYM_NUM_U] = {};
+    int elev[TCYM_NUM_E] = {};
+    Geom geo(int lev) const {
+        Geom q = g;
+        for (int k = 0; k < 3; k++) { q.n[k] = g.n[k] >> lev; q.a[k] = g.a[k]*(double)(1 << lev); }
+        q.V = (long)q.n[0]*q.n[1]*q.n[2];
+        return q;
+    }
+    // the geometry of the link slots u, u2 and field slots e, e2 (-1: unused)
+    // an entry point reads together; they must share one resolution
+    int at(Geom& q, int u, int e = -1, int u2 = -1, int e2 = -1) {
+        int lev = -1;
+        bool mixed = false;
+        auto take = [&](int l) { if (lev < 0) lev = l; else if (l != lev) mixed = true; };
+        if (u >= 0 && u < TCYM_NUM_U) take(ulev[u]);
+        if (u2 >= 0 && u2 < TCYM_NUM_U) take(ulev[u2]);
+        if (e >= 0 && e < TCYM_NUM_E) take(elev[e]);
+        if (e2 >= 0 && e2 < TCYM_NUM_E) take(elev[e2]);
+        if (mixed) return fail(TCYM_E_ARG, "slots of different resolutions (blocking levels)");
+        q = geo(lev < 0 ? 0 : lev);
+        return TCYM_OK;
+    }
     // device fields (sites padded to whole tiles) and host AoS arrays
-    long link_elems() const { return 3*tcym::padded_sites(g.V)*be->ns; }
-    long field_elems() const { return 3*tcym::padded_sites(g.V)*be->ne; }
-    long host_link_elems() const { return 3*g.V*be->ns; }
-    long host_field_elems() const { return 3*g.V*be->ne; }
+    long link_elems(const Geom& q) const { return 3*tcym::padded_sites(q.V)*be->ns; }
+    long field_elems(const Geom& q) const { return 3*tcym::padded_sites(q.V)*be->ne; }
+    long link_elems() const { return link_elems(g); }
+    long field_elems() const { return field_elems(g); }
 
     int useU(int slot) {
         if (slot < 0 || slot >= TCYM_NUM_U) return fail(TCYM_E_ARG, "link slot out of range");
@@ -95,18 +131,19 @@ struct tcym_ctx : tcym::DeviceQueue {
     }
     int edotb(int u, int e, int improved, double* out, bool map_out = false) {
         int r;
-        if ((r = useU(u)) || (r = useE(e))) return r;
+        Geom q;
+        if ((r = useU(u)) || (r = useE(e)) || (r = at(q, u, e))) return r;
         if (map_out && (r = alloc((void**)&map, sizeof(double)*(size_t)g.V, "hipMalloc(map)"))) return r;
         const int mx[1] = {0};
-        return reduce(1, mx, [&] { return be->edotb(L(), g, improved, U[u], E[e], part, map_out ? map : nullptr); }, out);
+        return reduce(1, mx, [&] { return be->edotb(L(), q, improved, U[u], E[e], part, map_out ? map : nullptr); }, out);
     }
-    double flow_dctau(double dtau) const {   // GRADIENTFLOW/GradientFlow.cpp:152
-        const double ac = g.a[0]*g.a[1]*g.a[2];
-        return dtau*pow(ac, 2.0/3.0)/(g.aScale*g.aScale);
+    static double flow_dctau(const Geom& q, double dtau) {   // GRADIENTFLOW/GradientFlow.cpp:152
+        const double ac = q.a[0]*q.a[1]*q.a[2];
+        return dtau*pow(ac, 2.0/3.0)/(q.aScale*q.aScale);
     }
-    void flow_cu(double dtau, double* cU) const {  // GradientFlow.cpp:109-113
-        const double ac = g.a[0]*g.a[1]*g.a[2], D = dtau*pow(ac, 2.0/3.0);
-        for (int mu = 0; mu < 3; mu++) cU[mu] = -(g.a[mu]*g.a[mu]*D)/(ac*g.aScale);
+    static void flow_cu(const Geom& q, double dtau, double* cU) {  // GradientFlow.cpp:109-113
+        const double ac = q.a[0]*q.a[1]*q.a[2], D = dtau*pow(ac, 2.0/3.0);
+        for (int mu = 0; mu < 3; mu++) cU[mu] = -(q.a[mu]*q.a[mu]*D)/(ac*q.aScale);
     }
     int ncsdot(int u, int e, double* out) {
         double v;
@@ -148,8 +185,11 @@ int tcym_create(const tcym_params* p, tcym_ctx** out) {
     c->be = p->nc == 2 ? tcym::make_backend_su2(f32) : tcym::make_backend_su3(f32);
     c->be64 = f32 ? (p->nc == 2 ? tcym::make_backend_su2(0) : tcym::make_backend_su3(0)) : c->be;
     int r = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
-    const long nblk = tcym::walk_shape(c->g).blocks;
-    if (!r) r = c->alloc((void**)&c->part, sizeof(double)*(size_t)nblk*8, "hipMalloc(partials)");
+    long nblk = 0;
+    for (int lev = 0; lev < 8 && c->geo(lev).n[0] >= 1 && c->geo(lev).n[1] >= 1 && c->geo(lev).n[2] >= 1; lev++)
+        nblk = std::max(nblk, tcym::walk_shape(c->geo(lev)).blocks);
+    // partials: up to 8 quantities per site block, or 3 per link block (3 per site block)
+    if (!r) r = c->alloc((void**)&c->part, sizeof(double)*(size_t)nblk*9, "hipMalloc(partials)");
     if (!r) r = c->alloc((void**)&c->red, sizeof(double)*16, "hipMalloc(results)");
     if (!r) r = c->hip(hipHostMalloc((void**)&c->red_host, sizeof(double)*16, hipHostMallocDefault), "hipHostMalloc");
     if (!r) r = c->alloc((void**)&c->stage, sizeof(double)*(size_t)(3*c->g.V*std::max(c->be->ns, c->be->ne)), "hipMalloc(stage)");
@@ -183,32 +223,36 @@ int tcym_set_links(tcym_ctx* c, int slot, const double* U) {
     if (!c || !U) return TCYM_E_ARG;
     int r = c->useU(slot);
     if (r) return r;
-    if ((r = c->hip(hipMemcpyAsync(c->stage, U, sizeof(double)*(size_t)c->host_link_elems(), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(links)"))) return r;
+    c->ulev[slot] = 0;
+    if ((r = c->hip(hipMemcpyAsync(c->stage, U, sizeof(double)*(size_t)(3*c->g.V*c->be->ns), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(links)"))) return r;
     c->be->pack_links(c->L(), c->g, c->stage, c->U[slot]);
     return c->sync();
 }
 int tcym_get_links(tcym_ctx* c, int slot, double* U) {
     if (!c || !U) return TCYM_E_ARG;
     int r = c->useU(slot);
-    if (r) return r;
-    c->be->unpack_links(c->L(), c->g, c->U[slot], c->stage);
-    if ((r = c->hip(hipMemcpyAsync(U, c->stage, sizeof(double)*(size_t)c->host_link_elems(), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(links)"))) return r;
+    Geom q;
+    if (r || (r = c->at(q, slot))) return r;
+    c->be->unpack_links(c->L(), q, c->U[slot], c->stage);
+    if ((r = c->hip(hipMemcpyAsync(U, c->stage, sizeof(double)*(size_t)(3*q.V*c->be->ns), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(links)"))) return r;
     return c->sync();
 }
 int tcym_set_efield(tcym_ctx* c, int slot, const double* E) {
     if (!c || !E) return TCYM_E_ARG;
     int r = c->useE(slot);
     if (r) return r;
-    if ((r = c->hip(hipMemcpyAsync(c->stage, E, sizeof(double)*(size_t)c->host_field_elems(), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(efield)"))) return r;
+    c->elev[slot] = 0;
+    if ((r = c->hip(hipMemcpyAsync(c->stage, E, sizeof(double)*(size_t)(3*c->g.V*c->be->ne), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(efield)"))) return r;
     c->be->pack_efield(c->L(), c->g, c->stage, c->E[slot]);
     return c->sync();
 }
 int tcym_get_efield(tcym_ctx* c, int slot, double* E) {
     if (!c || !E) return TCYM_E_ARG;
     int r = c->useE(slot);
-    if (r) return r;
-    c->be->unpack_efield(c->L(), c->g, c->E[slot], c->stage);
-    if ((r = c->hip(hipMemcpyAsync(E, c->stage, sizeof(double)*(size_t)c->host_field_elems(), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(efield)"))) return r;
+    Geom q;
+    if (r || (r = c->at(q, -1, slot))) return r;
+    c->be->unpack_efield(c->L(), q, c->E[slot], c->stage);
+    if ((r = c->hip(hipMemcpyAsync(E, c->stage, sizeof(double)*(size_t)(3*q.V*c->be->ne), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(efield)"))) return r;
     return c->sync();
 }
 int tcym_copy_links(tcym_ctx* c, int dst, int src) {
@@ -216,7 +260,8 @@ int tcym_copy_links(tcym_ctx* c, int dst, int src) {
     int r;
     if ((r = c->useU(dst)) || (r = c->useU(src))) return r;
     if (dst == src) return TCYM_OK;
-    c->be->copy(c->L(), c->link_elems(), c->U[src], c->U[dst]);
+    c->ulev[dst] = c->ulev[src];
+    c->be->copy(c->L(), c->link_elems(c->geo(c->ulev[src])), c->U[src], c->U[dst]);
     return c->sync();
 }
 int tcym_copy_efield(tcym_ctx* c, int dst, int src) {
@@ -224,7 +269,8 @@ int tcym_copy_efield(tcym_ctx* c, int dst, int src) {
     int r;
     if ((r = c->useE(dst)) || (r = c->useE(src))) return r;
     if (dst == src) return TCYM_OK;
-    c->be->copy(c->L(), c->field_elems(), c->E[src], c->E[dst]);
+    c->elev[dst] = c->elev[src];
+    c->be->copy(c->L(), c->field_elems(c->geo(c->elev[src])), c->E[src], c->E[dst]);
     return c->sync();
 }
 int tcym_zero_efield(tcym_ctx* c, int slot) {
@@ -238,6 +284,7 @@ int tcym_unit_links(tcym_ctx* c, int slot) {
     if (!c) return TCYM_E_ARG;
     int r = c->useU(slot);
     if (r) return r;
+    c->ulev[slot] = 0;
     c->be->random_links(c->L(), c->g, 0, 0.0, c->U[slot]);   // exp(0) = 1
     return c->sync();
 }
@@ -245,6 +292,7 @@ int tcym_random_links(tcym_ctx* c, int slot, uint64_t seed, double sigma) {
     if (!c) return TCYM_E_ARG;
     int r = c->useU(slot);
     if (r) return r;
+    c->ulev[slot] = 0;
     c->be->random_links(c->L(), c->g, seed, sigma, c->U[slot]);
     return c->sync();
 }
@@ -252,6 +300,7 @@ int tcym_random_efield(tcym_ctx* c, int slot, uint64_t seed, double sigma) {
     if (!c) return TCYM_E_ARG;
     int r = c->useE(slot);
     if (r) return r;
+    c->elev[slot] = 0;
     c->be->random_efield(c->L(), c->g, seed, 1u, 0u, sigma, c->E[slot]);
     return c->sync();
 }
@@ -259,8 +308,16 @@ int tcym_noise(tcym_ctx* c, int xi, uint64_t seed, uint32_t step, double stddev)
     if (!c) return TCYM_E_ARG;
     int r = c->useE(xi);
     if (r) return r;
+    c->elev[xi] = 0;
     c->be->random_efield(c->L(), c->g, seed, 2u, step, stddev, c->E[xi]);
     return c->sync();
+}
+int tcym_level(tcym_ctx* c, int u, int e, int* level, int N[3]) {
+    if (!c || (u >= 0) == (e >= 0) || u >= TCYM_NUM_U || e >= TCYM_NUM_E) return TCYM_E_ARG;
+    const int lev = u >= 0 ? c->ulev[u] : c->elev[e];
+    if (level) *level = lev;
+    if (N) for (int k = 0; k < 3; k++) N[k] = c->geo(lev).n[k];
+    return TCYM_OK;
 }
 
 // ---------------------------------------------------------------- updates
@@ -272,10 +329,12 @@ static int run_update(tcym_ctx* c, int mode, int u, int e, int xi, int nsteps, d
     if (mode != 3 && e < 0) return c->fail(TCYM_E_ARG, "E slot required");
     if (mode == 2 && (r = c->useE(xi))) return r;
     if (nsteps < 0) return c->fail(TCYM_E_ARG, "nsteps < 0");
+    Geom q;
+    if ((r = c->at(q, u, e, -1, mode == 2 ? xi : -1))) return r;
     void* E = e >= 0 ? c->E[e] : nullptr;
     const void* X = mode == 2 ? c->E[xi] : nullptr;
     for (int k = 0; k < nsteps; k++) {
-        c->be->update(c->L(), c->g, mode, true, write_e, dt, damping, noise_sd, seed, step0 + (uint32_t)k,
+        c->be->update(c->L(), q, mode, true, write_e, dt, damping, noise_sd, seed, step0 + (uint32_t)k,
                       c->U[u], c->Uscr, E, X);
         std::swap(c->U[u], c->Uscr);
         if (c->last != hipSuccess) return TCYM_E_HIP;
@@ -290,18 +349,20 @@ int tcym_leapfrog(tcym_ctx* c, int u, int e, int nsteps, double dt) {
 int tcym_update_efield(tcym_ctx* c, int u, int e, double dt) {
     if (!c) return TCYM_E_ARG;
     int r;
-    if ((r = c->useU(u)) || (r = c->useE(e))) return r;
-    c->be->update(c->L(), c->g, 0, false, true, dt, 0.0, 0.0, 0, 0, c->U[u], nullptr, c->E[e], nullptr);
+    Geom q;
+    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->at(q, u, e))) return r;
+    c->be->update(c->L(), q, 0, false, true, dt, 0.0, 0.0, 0, 0, c->U[u], nullptr, c->E[e], nullptr);
     return c->sync();
 }
 int tcym_update_links(tcym_ctx* c, int u, int e, double dt) {
     if (!c) return TCYM_E_ARG;
     int r;
-    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->scratch())) return r;
-    const double ac = c->g.a[0]*c->g.a[1]*c->g.a[2];
+    Geom q;
+    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->scratch()) || (r = c->at(q, u, e))) return r;
+    const double ac = q.a[0]*q.a[1]*q.a[2];
     double cU[3];
-    for (int mu = 0; mu < 3; mu++) cU[mu] = -dt*c->g.a[mu]*c->g.a[mu]/ac;   // DYNAMICS/Dynamics.cpp:102-104
-    c->be->link_update(c->L(), c->g, cU, c->U[u], c->Uscr, c->E[e]);
+    for (int mu = 0; mu < 3; mu++) cU[mu] = -dt*q.a[mu]*q.a[mu]/ac;   // DYNAMICS/Dynamics.cpp:102-104
+    c->be->link_update(c->L(), q, cU, c->U[u], c->Uscr, c->E[e]);
     std::swap(c->U[u], c->Uscr);
     return c->sync();
 }
@@ -310,6 +371,40 @@ int tcym_langevin(tcym_ctx* c, int u, int e, int nsteps, double dt, double dampi
     if (!c) return TCYM_E_ARG;
     if (!(T >= 0) || !(damping >= 0)) return c->fail(TCYM_E_ARG, "negative temperature or damping");
     return run_update(c, 1, u, e, -1, nsteps, dt, damping, sqrt(2.0*damping*T), seed, step0, true);
+}
+int tcym_evolve_ncs(tcym_ctx* c, int mode, int u, int e, int nsteps, double dt, double damping, double T,
+                    uint64_t seed, uint32_t step0, double* dncs) {
+    if (!c || !dncs) return TCYM_E_ARG;
+    if (mode != 0 && mode != 1) return c->fail(TCYM_E_ARG, "evolve_ncs: mode 0 (leapfrog) or 1 (Langevin)");
+    if (mode == 1 && (!(T >= 0) || !(damping >= 0))) return c->fail(TCYM_E_ARG, "negative temperature or damping");
+    if (nsteps < 0) return c->fail(TCYM_E_ARG, "nsteps < 0");
+    int r;
+    Geom q;
+    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->scratch()) || (r = c->at(q, u, e))) return r;
+    const double noise = mode == 1 ? sqrt(2.0*damping*T) : 0.0;
+    const int mx = 0;
+    double* eb = c->red + 8;     // E.B of U_n and U_n+1
+    double* acc = c->red + 10;   // running sum
+    const double zero = 0.0;
+    if ((r = c->hip(hipMemcpyAsync(acc, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(acc)"))) return r;
+    const double half_dt = 0.5*dt;
+    for (int k = 0; k < nsteps; k++) {
+        c->be->update(c->L(), q, mode, true, true, dt, damping, noise, seed, step0 + (uint32_t)k, c->U[u], c->Uscr, c->E[e], nullptr);
+        std::swap(c->U[u], c->Uscr);     // Uscr now holds U_n
+        long nb = c->be->edotb(c->L(), q, 1, c->Uscr, c->E[e], c->part, nullptr);
+        tcym::final_reduce(c->L(), c->part, nb, 1, &mx, eb);
+        nb = c->be->edotb(c->L(), q, 1, c->U[u], c->E[e], c->part, nullptr);
+        tcym::final_reduce(c->L(), c->part, nb, 1, &mx, eb + 1);
+        c->launch("hot_ncs", [&](hipStream_t st) { hipLaunchKernelGGL(k_hot_ncs, dim3(1), dim3(64), 0, st, (const double*)eb, acc, half_dt); });
+        if (c->last != hipSuccess) return TCYM_E_HIP;
+    }
+    double h = 0.0;
+    if ((r = c->hip(hipMemcpyAsync(c->red_host, acc, sizeof(double), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync(acc)"))) return r;
+    if ((r = c->sync())) return r;
+    h = c->red_host[0];
+    if (!isfinite(h)) return c->fail(TCYM_E_NONFINITE, "non-finite N_CS integral");
+    *dncs += h;
+    return TCYM_OK;
 }
 int tcym_langevin_xi(tcym_ctx* c, int u, int e, int xi, double dt, double damping) {
     if (!c) return TCYM_E_ARG;
@@ -320,16 +415,17 @@ int tcym_flow(tcym_ctx* c, int u, int e, int nsteps, double dtau, int calibrate,
     if (!calibrate) return run_update(c, 3, u, e, -1, nsteps, dtau, 0.0, 0.0, 0, 0, e >= 0);
     int r;
     if (e < 0) return c->fail(TCYM_E_ARG, "calibration flow needs an E slot");
-    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->scratch())) return r;
-    const double D = c->flow_dctau(dtau);
+    Geom q;
+    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->scratch()) || (r = c->at(q, u, e))) return r;
+    const double D = tcym_ctx::flow_dctau(q, dtau);
     double cU[3], acc = 0.0, v;
-    c->flow_cu(dtau, cU);
+    tcym_ctx::flow_cu(q, dtau, cU);
     for (int k = 0; k < nsteps; k++) {
         // GradientFlow::CalibrationUpdate (GRADIENTFLOW/GradientFlow.cpp:168-188)
-        c->be->update(c->L(), c->g, 3, false, true, dtau, 0.0, 0.0, 0, 0, c->U[u], nullptr, c->E[e], nullptr);
+        c->be->update(c->L(), q, 3, false, true, dtau, 0.0, 0.0, 0, 0, c->U[u], nullptr, c->E[e], nullptr);
         if ((r = c->ncsdot(u, e, &v))) return r;
         acc += 0.5*D*v;
-        c->be->link_update(c->L(), c->g, cU, c->U[u], c->Uscr, c->E[e]);
+        c->be->link_update(c->L(), q, cU, c->U[u], c->Uscr, c->E[e]);
         std::swap(c->U[u], c->Uscr);
         if ((r = c->ncsdot(u, e, &v))) return r;
         acc += 0.5*D*v;
@@ -348,8 +444,9 @@ int tcym_cool(tcym_ctx* c, int u_hot, int u_cold, int nsteps, double dtau) {
 int tcym_reunitarize(tcym_ctx* c, int u) {
     if (!c) return TCYM_E_ARG;
     int r = c->useU(u);
-    if (r) return r;
-    c->be->reunitarize(c->L(), c->g, c->U[u]);
+    Geom q;
+    if (r || (r = c->at(q, u))) return r;
+    c->be->reunitarize(c->L(), q, c->U[u]);
     return c->sync();
 }
 
@@ -366,46 +463,62 @@ int tcym_edotb_map(tcym_ctx* c, int u, int e, double* map) {
     if (!c || !map) return TCYM_E_ARG;
     double tot;
     int r = c->edotb(u, e, 1, &tot, true);
-    if (r) return r;
-    if ((r = c->hip(hipMemcpyAsync(map, c->map, sizeof(double)*(size_t)c->g.V, hipMemcpyDeviceToHost, c->stream), "copy map"))) return r;
+    Geom q;
+    if (r || (r = c->at(q, u, e))) return r;
+    if ((r = c->hip(hipMemcpyAsync(map, c->map, sizeof(double)*(size_t)q.V, hipMemcpyDeviceToHost, c->stream), "copy map"))) return r;
     return c->sync();
 }
 int tcym_bulk(tcym_ctx* c, int u, int e, double out[6]) {
     if (!c || !out) return TCYM_E_ARG;
     int r;
-    if ((r = c->useU(u)) || (r = c->useE(e))) return r;
+    Geom q;
+    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->at(q, u, e))) return r;
     const int mx[6] = {0, 0, 0, 0, 0, 0};
     double s[6];
-    if ((r = c->reduce(6, mx, [&] { return c->be->bulk(c->L(), c->g, c->U[u], c->E[e], c->part); }, s))) return r;
-    for (int k = 0; k < 6; k++) out[k] = s[k]/(double)c->g.V;
+    if ((r = c->reduce(6, mx, [&] { return c->be->bulk(c->L(), q, c->U[u], c->E[e], c->part); }, s))) return r;
+    for (int k = 0; k < 6; k++) out[k] = s[k]/(double)q.V;
     return TCYM_OK;
 }
 int tcym_gauss(tcym_ctx* c, int u, int e, double* smax, double* rms) {
     if (!c) return TCYM_E_ARG;
     int r;
-    if ((r = c->useU(u)) || (r = c->useE(e))) return r;
+    Geom q;
+    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->at(q, u, e))) return r;
     const int mx[2] = {1, 0};
     double s[2];
-    if ((r = c->reduce(2, mx, [&] { return c->be->gauss(c->L(), c->g, c->U[u], c->E[e], c->part, nullptr); }, s))) return r;
+    if ((r = c->reduce(2, mx, [&] { return c->be->gauss(c->L(), q, c->U[u], c->E[e], c->part, nullptr); }, s))) return r;
     if (smax) *smax = s[0];
-    if (rms) *rms = sqrt(s[1]/((double)c->be->ne*c->g.V));
+    if (rms) *rms = sqrt(s[1]/((double)c->be->ne*q.V));
+    return TCYM_OK;
+}
+int tcym_hard_scales(tcym_ctx* c, int u, double out[3]) {
+    if (!c || !out) return TCYM_E_ARG;
+    int r = c->useU(u);
+    Geom q;
+    if (r || (r = c->at(q, u))) return r;
+    const int mx[3] = {0, 0, 0};
+    double s[3];
+    if ((r = c->reduce(3, mx, [&] { return c->be->hard_scales(c->L(), q, c->U[u], c->part); }, s))) return r;
+    for (int k = 0; k < 3; k++) out[k] = s[k]/(double)q.V;   // HardScales.cpp:98
     return TCYM_OK;
 }
 int tcym_unitarity(tcym_ctx* c, int u, double* umax, double* rms) {
     if (!c) return TCYM_E_ARG;
     int r = c->useU(u);
-    if (r) return r;
+    Geom q;
+    if (r || (r = c->at(q, u))) return r;
     const int mx[2] = {1, 0};
     double s[2];
-    if ((r = c->reduce(2, mx, [&] { return c->be->unitarity(c->L(), c->g, c->U[u], c->part); }, s))) return r;
+    if ((r = c->reduce(2, mx, [&] { return c->be->unitarity(c->L(), q, c->U[u], c->part); }, s))) return r;
     if (umax) *umax = s[0];
-    if (rms) *rms = sqrt(s[1]/(3.0*c->g.V));
+    if (rms) *rms = sqrt(s[1]/(3.0*q.V));
     return TCYM_OK;
 }
 int tcym_gauss_restore(tcym_ctx* c, int u, int e, int maxiter, double tol, int* iters, double* final_max) {
     if (!c) return TCYM_E_ARG;
     int r;
-    if ((r = c->useU(u)) || (r = c->useE(e))) return r;
+    Geom q;
+    if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->at(q, u, e))) return r;
     if ((r = c->alloc((void**)&c->G64, sizeof(double)*(size_t)(c->g.V*c->be->ne), "hipMalloc(G)"))) return r;
     double* Uw;
     double* Ew;
@@ -414,15 +527,15 @@ int tcym_gauss_restore(tcym_ctx* c, int u, int e, int maxiter, double tol, int* 
             return c->fail(TCYM_E_UNSUPPORTED, "tcym_gauss_restore: the FP64 copy of the links would exceed 4 GiB");
         if ((r = c->alloc((void**)&c->U64, sizeof(double)*(size_t)c->link_elems(), "hipMalloc(U64)"))) return r;
         if ((r = c->alloc((void**)&c->E64, sizeof(double)*(size_t)c->field_elems(), "hipMalloc(E64)"))) return r;
-        c->be->to_f64(c->L(), c->g, c->be->ns, c->U[u], c->U64);
-        c->be->to_f64(c->L(), c->g, c->be->ne, c->E[e], c->E64);
+        c->be->to_f64(c->L(), q, c->be->ns, c->U[u], c->U64);
+        c->be->to_f64(c->L(), q, c->be->ne, c->E[e], c->E64);
         Uw = c->U64; Ew = c->E64;
     } else {
         Uw = (double*)c->U[u]; Ew = (double*)c->E[e];
     }
     const int mx[2] = {1, 0};
     double s[2];
-    auto dev = [&] { return c->be64->gauss(c->L(), c->g, Uw, Ew, c->part, c->G64); };
+    auto dev = [&] { return c->be64->gauss(c->L(), q, Uw, Ew, c->part, c->G64); };
     if ((r = c->reduce(2, mx, dev, s))) return r;
     double smax = s[0];
     int n = 0;
@@ -431,13 +544,13 @@ int tcym_gauss_restore(tcym_ctx* c, int u, int e, int maxiter, double tol, int* 
         for (int k = 0; k < 3; k++) {
             if (k < 2) dev();
             else if ((r = c->reduce(2, mx, dev, s))) return r;
-            c->be64->gauss_correct(c->L(), c->g, Uw, Ew, c->G64, gam[k]);
+            c->be64->gauss_correct(c->L(), q, Uw, Ew, c->G64, gam[k]);
         }
         smax = s[0];
         n++;
         if (c->last != hipSuccess) return TCYM_E_HIP;
     }
-    if (c->prec == TCYM_FP32) c->be->from_f64(c->L(), c->g, c->be->ne, c->E64, c->E[e]);
+    if (c->prec == TCYM_FP32) c->be->from_f64(c->L(), q, c->be->ne, c->E64, c->E[e]);
     if ((r = c->sync())) return r;
     if (iters) *iters = n;
     if (final_max) *final_max = smax;
@@ -447,8 +560,11 @@ int tcym_gauss_restore(tcym_ctx* c, int u, int e, int maxiter, double tol, int* 
 int tcym_geodesic(tcym_ctx* c, int uold, int unew, int umid, int emid, double cc) {
     if (!c) return TCYM_E_ARG;
     int r;
+    Geom q;
     if ((r = c->useU(uold)) || (r = c->useU(unew)) || (r = c->useU(umid)) || (r = c->useE(emid))) return r;
-    c->be->geodesic(c->L(), c->g, cc, c->U[uold], c->U[unew], c->U[umid], c->E[emid]);
+    if ((r = c->at(q, uold, -1, unew))) return r;
+    c->ulev[umid] = c->elev[emid] = c->ulev[uold];
+    c->be->geodesic(c->L(), q, cc, c->U[uold], c->U[unew], c->U[umid], c->E[emid]);
     return c->sync();
 }
 int tcym_delta_ncs_cooling(tcym_ctx* c, int uold, int unew, double* out) {
@@ -486,9 +602,103 @@ int tcym_cooling_update(tcym_ctx* c, int u_hot, int nsteps, double dtau, int mea
 int tcym_vacuum_ncs(tcym_ctx* c, int u, double* out) {
     if (!c || !out) return TCYM_E_ARG;
     int r = c->useU(u);
-    if (r) return r;
+    Geom q;
+    if (r || (r = c->at(q, u))) return r;
     const int mx[1] = {0};
-    return c->reduce(1, mx, [&] { return c->be->vacuum_ncs(c->L(), c->g, c->U[u], c->part); }, out);
+    return c->reduce(1, mx, [&] { return c->be->vacuum_ncs(c->L(), q, c->U[u], c->part); }, out);
+}
+
+// ---------------------------------------------------------------- blocking and calibration
+int tcym_block(tcym_ctx* c, int u, int e) {
+    if (!c) return TCYM_E_ARG;
+    int r;
+    if ((r = c->useU(u)) || (r = c->scratch())) return r;
+    if (e >= 0 && (r = c->useE(e))) return r;
+    const Geom q = c->geo(c->ulev[u]);
+    for (int k = 0; k < 3; k++)
+        if (q.n[k] % 2 != 0 || q.n[k]/2 < 2) return c->fail(TCYM_E_ARG, "blocking needs even extents of at least 4");
+    // Block(GaugeLinks**) (LATTICE/GaugeLinks.cpp:125-151) into the scratch buffer
+    c->be->block_links(c->L(), q, c->U[u], c->Uscr);
+    std::swap(c->U[u], c->Uscr);
+    c->ulev[u]++;
+    if (e >= 0) {
+        // Block(ElectricFields**) sets the coarse field to zero (LATTICE/ElectricFields.cpp:113-127)
+        if ((r = c->hip(hipMemsetAsync(c->E[e], 0, (size_t)c->field_elems()*c->be->elsize, c->stream), "zero efield"))) return r;
+        c->elev[e] = c->ulev[u];
+    }
+    return c->sync();
+}
+// flow steps step0+1 .. nsteps of Cooling::Perform with blocking (GRADIENTFLOW/Cooling.cpp:69-147);
+// calibrate: GradientFlow::CalibrationUpdate with the Delta N_CS sum and the log rows
+static int perform_cooling(tcym_ctx* c, int u, int e, int step0, int nsteps, double dtau, int bfreq, int maxb,
+                           int* nblock, int calibrate, double ctau, double* dncs, double* log) {
+    int r;
+    double acc = 0.0;
+    auto log_row = [&](double* row) -> int {
+        // CoolingTime, Bulk::MAGNETIC, Bulk::ELECTRIC, DeltaNCsCooling, VacuumEstimator::NCS (Cooling.cpp:85-111)
+        Geom q;
+        int rr;
+        if ((rr = c->at(q, u, e))) return rr;
+        double b[6], vac;
+        if ((rr = tcym_bulk(c, u, e, b)) || (rr = tcym_vacuum_ncs(c, u, &vac))) return rr;
+        const double a3 = q.a[0]*q.a[1]*q.a[2];
+        double el = 0.0, mg = 0.0;
+        for (int mu = 0; mu < 3; mu++) {
+            const double cc = q.a[mu]*q.aScale*q.aScale/a3, cb = cc*cc;   // BulkObservables.cpp:17-25 (Minkowski)
+            el += cb*b[mu];
+            mg += cb*b[3 + mu];
+        }
+        row[0] = ctau; row[1] = 0.5*mg; row[2] = 0.5*el; row[3] = acc; row[4] = vac;
+        return TCYM_OK;
+    };
+    double* row = log;
+    if (calibrate && row) { if ((r = log_row(row))) return r; row += 5; }
+    for (int k = step0 + 1; k <= nsteps; k++) {
+        Geom q;
+        if ((r = c->at(q, u, calibrate ? e : -1))) return r;
+        const double D = tcym_ctx::flow_dctau(q, dtau);
+        if (calibrate) {
+            double d = 0.0;
+            if ((r = tcym_flow(c, u, e, 1, dtau, 1, &d))) return r;
+            acc += d;
+        } else if ((r = run_update(c, 3, u, -1, -1, 1, dtau, 0.0, 0.0, 0, 0, false))) {
+            return r;
+        }
+        ctau += D;
+        if (calibrate && row) { if ((r = log_row(row))) return r; row += 5; }
+        if (bfreq > 0 && *nblock < maxb) {
+            long every = bfreq;
+            for (int l = 0; l < *nblock; l++) every *= 8;    // BlockFrequency 8^BlockingNumber (Cooling.cpp:123)
+            if (k % every == 0) {
+                if ((r = tcym_block(c, u, calibrate ? e : -1))) return r;
+                (*nblock)++;
+            }
+        }
+    }
+    if (dncs) *dncs = acc;
+    return c->sync();
+}
+int tcym_cool_blocked(tcym_ctx* c, int u_hot, int u_cold, int nsteps, double dtau, int block_frequency,
+                      int max_blocking, int* nblocked) {
+    if (!c) return TCYM_E_ARG;
+    if (nsteps < 0 || block_frequency < 0 || max_blocking < 0) return c->fail(TCYM_E_ARG, "negative step or blocking count");
+    int r;
+    if ((r = tcym_copy_links(c, u_cold, u_hot))) return r;
+    int nb = 0;
+    if ((r = perform_cooling(c, u_cold, -1, 0, nsteps, dtau, block_frequency, max_blocking, &nb, 0, 0.0, nullptr, nullptr))) return r;
+    if (nblocked) *nblocked = nb;
+    return TCYM_OK;
+}
+int tcym_continue_cooling(tcym_ctx* c, int u_src, int u_work, int e_work, int step0, int nsteps, double dtau,
+                          double ctau0, int block_frequency, int max_blocking, int* nblocked, double* dncs, double* log) {
+    if (!c || !nblocked) return TCYM_E_ARG;
+    if (step0 < 0 || nsteps < step0 || block_frequency < 0 || max_blocking < 0) return c->fail(TCYM_E_ARG, "bad step or blocking counts");
+    int r;
+    // Cooling::Setup: copy the links, E zero at their resolution (GRADIENTFLOW/Cooling.cpp:38-45)
+    if ((r = tcym_copy_links(c, u_work, u_src)) || (r = c->useE(e_work))) return r;
+    if ((r = c->hip(hipMemsetAsync(c->E[e_work], 0, (size_t)c->field_elems()*c->be->elsize, c->stream), "zero efield"))) return r;
+    c->elev[e_work] = c->ulev[u_work];
+    return perform_cooling(c, u_work, e_work, step0, nsteps, dtau, block_frequency, max_blocking, nblocked, 1, ctau0, dncs, log);
 }
 
 // ---------------------------------------------------------------- timing

@@ -80,11 +80,14 @@ struct Backend {
     virtual long bulk(const Launcher&, const Geom&, const void* U, const void* E, double* part) = 0;
     virtual long gauss(const Launcher&, const Geom&, const void* U, const void* E, double* part, double* G) = 0;
     virtual long unitarity(const Launcher&, const Geom&, const void* U, double* part) = 0;
+    virtual long hard_scales(const Launcher&, const Geom&, const void* U, double* part) = 0;   // 3 quantities per link tile
     virtual long vacuum_ncs(const Launcher&, const Geom&, const void* U, double* part) = 0;
     // Gauss-law restoration (FP64 buffers only)
     virtual void gauss_correct(const Launcher&, const Geom&, const double* U, double* E, const double* G, double gamma) = 0;
     virtual void geodesic(const Launcher&, const Geom&, double c, const void* Uold, const void* Unew, void* Umid, void* Emid) = 0;
     virtual void reunitarize(const Launcher&, const Geom&, void* U) = 0;
+    // links of the fine lattice g -> the lattice of half the extent (Uc distinct from Uf)
+    virtual void block_links(const Launcher&, const Geom& g, const void* Uf, void* Uc) = 0;
 
     // batches of lattices (tcym_batch.hip): one persistent launch per pass
     // over every lattice of the batch, same per-link / per-site arithmetic as

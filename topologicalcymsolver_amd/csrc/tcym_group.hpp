@@ -547,15 +547,27 @@ struct Philox {
         const uint64_t x = ((uint64_t)hi << 21) | (uint64_t)(lo >> 11);
         return ((double)x + 1.0)*(1.0/9007199254740992.0);
     }
-    // n normals for (seed, stream, step, link): counter (link, step, stream, j)
-    __device__ static __forceinline__ void normals(uint64_t seed, uint32_t stream, uint32_t step, uint32_t link, int n, double* z) {
+    // n normals for (seed, stream, step, link): counter (link, step, stream, j),
+    // Box-Muller on two 53-bit uniforms.  T = double for the FP64 form (and for
+    // the initial conditions of both forms); T = float evaluates the logarithm
+    // and the sine/cosine in FP32 from the same uniforms (a tiny u1 keeps its
+    // magnitude in FP32, so the tails are those of the FP64 draw) at a quarter
+    // of the cost; the FP32 form's noise then differs from the FP64 form's by
+    // FP32 rounding.
+    template <typename T = double>
+    __device__ static __forceinline__ void normals(uint64_t seed, uint32_t stream, uint32_t step, uint32_t link, int n, T* z) {
         for (int j = 0; 2*j < n; j++) {
             uint32_t c[4] = {link, step, stream, (uint32_t)j};
             gen(c, (uint32_t)seed, (uint32_t)(seed >> 32));
             const double u1 = u53(c[0], c[1]), u2 = u53(c[2], c[3]);
-            const double R = __builtin_sqrt(-2.0*log(u1)), t = 6.283185307179586476925286766559*u2;
-            double sn, cs;
-            sincos(t, &sn, &cs);
+            T sn, cs, R;
+            if constexpr (sizeof(T) == 8) {
+                R = __builtin_sqrt(-2.0*log(u1));
+                sincos(6.283185307179586476925286766559*u2, &sn, &cs);
+            } else {
+                R = __builtin_sqrtf(-2.0f*logf((float)u1));
+                sincosf(6.2831853071795864769f*(float)u2, &sn, &cs);
+            }
             z[2*j] = R*cs;
             if (2*j + 1 < n) z[2*j + 1] = R*sn;
         }
