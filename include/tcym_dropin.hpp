@@ -7,9 +7,9 @@
 //   Dynamics::Update(GaugeLinks*, ElectricFields*)           DYNAMICS/Dynamics.cpp:204
 //   LangevinDynamics::UpdateFields(GaugeLinks*, ElectricFields*)  DYNAMICS/LangevinDynamics.cpp:300
 //   GradientFlow::StandardUpdate / CalibrationUpdate           GRADIENTFLOW/GradientFlow.cpp:150,168
-//   Cooling::CoolNSave                                         GRADIENTFLOW/Cooling.cpp:156
+//   Cooling::CoolNSave / ContinueCooling                       GRADIENTFLOW/Cooling.cpp:156,181
 //   ChernSimonsNumber::NCsDot                                  TOPOLOGY/ChernSimonsDerivative.cpp:31
-//   ChernSimonsNumber::CoolingMethod::{Start,Update}           TOPOLOGY/CoolingMethod.cpp:127,143
+//   ChernSimonsNumber::CoolingMethod::{Start,Update,Calibrate} TOPOLOGY/CoolingMethod.cpp:127,143,171
 //   ImprovedOperators::ComputeEDotB                            CALC/ChernSimonsOperators.cpp:384
 //   Observables::{Bulk,GaussLaw,Unitarity}, EdotBMap::CreateMap OBSERVABLES/*.cpp
 //   GaussLawRestoration::Restore                               GAUSSLAWFIXING/GaussRestore.cpp:131
@@ -33,6 +33,7 @@
 #include <stdexcept>
 #include <string>
 #include <fstream>
+#include <sstream>
 
 #include "tcym.h"
 
@@ -174,6 +175,10 @@ public:
         for (int s = 0; s < TCYM_NUM_U; s++) flush_links(s);
         for (int s = 0; s < TCYM_NUM_E; s++) flush_field(s);
     }
+    // the device slot is about to be overwritten: download a device-newer host
+    // object it held and forget the binding
+    void release_links(int s) { flush_links(s); ub[s] = Bind(); }
+    void release_field(int s) { flush_field(s); eb[s] = Bind(); }
     void invalidate() {
         for (auto& b : ub) b.valid = false;
         for (auto& b : eb) b.valid = false;
@@ -310,10 +315,12 @@ inline INT& CoolingSteps() { static INT v = 0; return v; }
 inline void Reset() { cTau() = 0.0; CoolingSteps() = 0; }
 inline DOUBLE CoolingTime() { return cTau(); }
 inline DOUBLE dtau() { return SqrtDcTauOverSpacing()*SqrtDcTauOverSpacing(); }
-inline void advance(const GaugeLinks* U) {
-    cTau() += dtau()*std::pow(U->aCube, 2.0/3.0)/(Lattice::aScale()*Lattice::aScale());
+// CoolingSteps++ and cTau += DcTau of one flow step at spacings a (:150-163)
+inline void advance(const DOUBLE* a) {
+    cTau() += dtau()*std::pow(a[0]*a[1]*a[2], 2.0/3.0)/(Lattice::aScale()*Lattice::aScale());
     CoolingSteps()++;
 }
+inline void advance(const GaugeLinks* U) { advance(U->a); }
 inline void StandardUpdate(GaugeLinks* U, ElectricFields* E) {   // :150-165
     auto& d = tcym_dropin::dev(U);
     const int u = d.links(U, TCYM_U_COOL), e = d.field(E, TCYM_E_COOL);
@@ -334,34 +341,158 @@ inline void CalibrationUpdate(GaugeLinks* U, ElectricFields* E) {   // :168-188
 }
 }  // namespace GradientFlow
 
+namespace IO {
+// output directory of the calibration logs (IO/OutputManagement.cpp:3-12)
+inline std::string& OutputDirectory() { static std::string d; return d; }
+inline void SetOutputDirectory(const std::string& x) { OutputDirectory() = x + "/"; }
+}  // namespace IO
+
+namespace RandomNumberGenerator {
+// the seed that names the calibration logs (MISC/RNG/GSLRandomNumberGenerator.cpp:9,19)
+inline long& MySEED() { static long v = 0; return v; }
+}
+
+namespace ChernSimonsNumber {
+inline DOUBLE& DeltaNCsCalibration() { static DOUBLE v = 0.0; return v; }                   // TOPOLOGY/ChernSimonsDerivative.cpp:25
+inline DOUBLE& DeltaNCsPreviousCalibration() { static DOUBLE v = 0.0; return v; }
+inline DOUBLE& DeltaNCsRealTimeCalibration() { static DOUBLE v = 0.0; return v; }           // :28
+inline DOUBLE& DeltaNCsRealTimePreviousCalibration() { static DOUBLE v = 0.0; return v; }
+}  // namespace ChernSimonsNumber
+
 namespace Cooling {
-// Cooling::CoolNSave without blocking (GRADIENTFLOW/Cooling.cpp:156-179)
+// Cooling::BlockFrequency (GRADIENTFLOW/Cooling.cpp:22); 0 never blocks (the
+// reference would divide by zero)
+inline INT& BlockFrequency() { static INT v = 0; return v; }
+// Cooling::DynamicFields::BlockingNumber: blockings of the last cooled copy (:14)
+inline INT& BlockingNumber() { static INT v = 0; return v; }
+
+// GradientFlow::CoolingSteps / cTau over flow steps step0+1 .. nsteps of
+// Cooling::Perform (Cooling.cpp:92-141): step k flows at the spacing of the
+// blockings done before it; a is the spacing at nb blockings.
+inline void clock(const DOUBLE* a, int step0, int nsteps, int maxb, int nb) {
+    DOUBLE s[3] = {a[0], a[1], a[2]};
+    for (int k = step0 + 1; k <= nsteps; k++) {
+        GradientFlow::advance(s);
+        if (BlockFrequency() > 0 && nb < maxb) {
+            long every = BlockFrequency();
+            for (int l = 0; l < nb; l++) every *= 8;
+            if (k % every == 0) {
+                nb++;
+                for (int m = 0; m < 3; m++) s[m] *= 2;   // Block(GaugeLinks**) (LATTICE/GaugeLinks.cpp:125-151)
+            }
+        }
+    }
+}
+
+// Cooling::CoolNSave (GRADIENTFLOW/Cooling.cpp:156-179).  With blocking, UCold
+// becomes the coarse lattice, as the reference's Copy(UCold, U) sets its
+// extents, spacings and volume (LATTICE/GaugeLinks.cpp:91-107): its array must
+// hold the coarse links, which the fine allocation does.
 inline void CoolNSave(std::string, DOUBLE MaxCoolingSteps, INT MaxBlockingNumber, GaugeLinks* UHot, GaugeLinks* UCold) {
-    if (MaxBlockingNumber != 0) throw tcym_dropin::Error(TCYM_E_UNSUPPORTED, "blocking of cooled copies is not provided");
     auto& d = tcym_dropin::dev(UHot);
     GradientFlow::Reset();
-    d.check(tcym_cool(d.ctx, d.links(UHot, TCYM_U_HOT), TCYM_U_NEW, (int)MaxCoolingSteps, GradientFlow::dtau()));
-    for (int k = 0; k < (int)MaxCoolingSteps; k++) GradientFlow::advance(UHot);
+    int nb = 0;
+    d.check(tcym_cool_blocked(d.ctx, d.links(UHot, TCYM_U_HOT), TCYM_U_NEW, (int)MaxCoolingSteps, GradientFlow::dtau(),
+                              BlockFrequency(), MaxBlockingNumber, &nb));
+    clock(UHot->a, 0, (int)MaxCoolingSteps, MaxBlockingNumber, 0);
+    BlockingNumber() = nb;
     d.check(tcym_get_links(d.ctx, TCYM_U_NEW, UCold->U));
+    const int f = 1 << nb;
+    for (int m = 0; m < 3; m++) { UCold->N[m] = UHot->N[m]/f; UCold->a[m] = UHot->a[m]*f; }
+    UCold->Volume = UCold->N[0]*UCold->N[1]*UCold->N[2];
+    UCold->aCube = UCold->a[0]*UCold->a[1]*UCold->a[2];
+}
+
+// Cooling::ContinueCooling on device slot src of context d, whose spacing is
+// a (GRADIENTFLOW/Cooling.cpp:181-195): E of the working copy zero, calibration
+// flow from the current GradientFlow::CoolingSteps up to MaxCoolingSteps,
+// blocking counted on from BlockingNumber, and the log
+// "CoolingTime MAGNETIC ELECTRIC DeltaNCsCooling VacEst" written to
+// IO::OutputDirectory + fname + "ID" + MySEED + ".log" (Cooling.cpp:69-111).
+inline void ContinueCoolingSlot(const std::string& fname, DOUBLE MaxCoolingSteps, INT MaxBlockingNumber,
+                                tcym_dropin::Device& d, int src, const DOUBLE* a) {
+    const int step0 = GradientFlow::CoolingSteps(), nsteps = (int)MaxCoolingSteps;
+    if (nsteps < step0) return;
+    // the working slots are overwritten: save and release whatever host objects they held
+    d.release_links(TCYM_U_COOL);
+    d.release_field(TCYM_E_COOL);
+    const int nb0 = BlockingNumber();
+    int nb = nb0;
+    double dn = 0.0;
+    double* log = new double[(size_t)5*(nsteps - step0 + 1)];
+    const int r = tcym_continue_cooling(d.ctx, src, TCYM_U_COOL, TCYM_E_COOL, step0, nsteps, GradientFlow::dtau(),
+                                        GradientFlow::cTau(), BlockFrequency(), MaxBlockingNumber, &nb, &dn, log);
+    if (r != TCYM_OK) { delete[] log; d.check(r); }
+    std::ostringstream name;
+    name << IO::OutputDirectory() << fname << "ID" << RandomNumberGenerator::MySEED() << ".log";
+    std::ofstream out(name.str().c_str());
+    const double n0 = ChernSimonsNumber::DeltaNCsCooling();
+    for (int j = 0; j <= nsteps - step0; j++) {
+        const double* w = log + 5*j;
+        out << w[0] << " " << w[1] << " " << w[2] << " " << n0 + w[3] << " " << w[4] << std::endl;
+    }
+    delete[] log;
+    clock(a, step0, nsteps, MaxBlockingNumber, nb0);
+    BlockingNumber() = nb;
+    ChernSimonsNumber::DeltaNCsCooling() += dn;
+}
+inline void ContinueCooling(std::string fname, DOUBLE MaxCoolingSteps, INT MaxBlockingNumber, GaugeLinks* UHot) {
+    auto& d = tcym_dropin::dev(UHot);
+    d.release_links(TCYM_U_COOL);
+    ContinueCoolingSlot(fname, MaxCoolingSteps, MaxBlockingNumber, d, d.links(UHot, TCYM_U_AUX), UHot->a);
 }
 }  // namespace Cooling
 
 namespace ChernSimonsNumber {
 namespace CoolingMethod {
 inline INT& StandardCoolingMaxSteps() { static INT v = 128; return v; }   // SphaleronSimulation.cpp:230
+inline INT& CalibrationCoolingMaxSteps() { static INT v = 0; return v; }  // TOPOLOGY/CoolingMethod.cpp:7
+inline INT& StandardBlockingLevel() { static INT v = 0; return v; }       // :5
+inline INT& CalibrationBlockingLevel() { static INT v = 0; return v; }
+// Cooling::CoolNSave(GLinks::U -> UNew) on the device (:133,152)
+inline void cool_new(tcym_dropin::Device& d, GaugeLinks* UHot) {
+    GradientFlow::Reset();
+    int nb = 0;
+    d.check(tcym_cool_blocked(d.ctx, d.links(UHot, TCYM_U_HOT), TCYM_U_NEW, StandardCoolingMaxSteps(), GradientFlow::dtau(),
+                              Cooling::BlockFrequency(), StandardBlockingLevel(), &nb));
+    Cooling::clock(UHot->a, 0, StandardCoolingMaxSteps(), StandardBlockingLevel(), 0);
+    Cooling::BlockingNumber() = nb;
+}
 // CoolingMethod::Start (TOPOLOGY/CoolingMethod.cpp:127-137): cools GLinks::U into UNew, UOld <- UNew
 inline void Start(GaugeLinks* UHot) {
     auto& d = tcym_dropin::dev(UHot);
-    d.check(tcym_cooling_start(d.ctx, d.links(UHot, TCYM_U_HOT), StandardCoolingMaxSteps(), GradientFlow::dtau()));
+    cool_new(d, UHot);
+    d.check(tcym_copy_links(d.ctx, TCYM_U_OLD, TCYM_U_NEW));
 }
 // CoolingMethod::Update(Measure) (:143-165)
 inline void Update(GaugeLinks* UHot, INT Measure) {
     auto& d = tcym_dropin::dev(UHot);
     DeltaNCsCooling() = 0.0;
-    double dn = 0.0;
-    d.check(tcym_cooling_update(d.ctx, d.links(UHot, TCYM_U_HOT), StandardCoolingMaxSteps(), GradientFlow::dtau(),
-                                Measure ? 1 : 0, &dn));
-    if (Measure == 1) DeltaNCsCoolRealTime() += dn;
+    cool_new(d, UHot);
+    if (Measure == 1) {
+        double dn = 0.0;
+        d.check(tcym_delta_ncs_cooling(d.ctx, TCYM_U_OLD, TCYM_U_NEW, &dn));
+        DeltaNCsCoolRealTime() += dn;
+    }
+    d.check(tcym_copy_links(d.ctx, TCYM_U_OLD, TCYM_U_NEW));
+}
+// CoolingMethod::Calibrate (:171-195): cool UNew (= UOld after Start/Update)
+// on to the vacuum with CalibrationUpdate; UHot names the lattice
+inline void Calibrate(GaugeLinks* UHot) {
+    auto& d = tcym_dropin::dev(UHot);
+    DeltaNCsCooling() = 0.0;
+    DeltaNCsPreviousCalibration() = DeltaNCsCalibration();
+    DeltaNCsRealTimePreviousCalibration() = DeltaNCsRealTimeCalibration();
+    int lev = 0, n[3];
+    d.check(tcym_level(d.ctx, TCYM_U_OLD, -1, &lev, n));
+    DOUBLE a[3];
+    for (int m = 0; m < 3; m++) a[m] = UHot->a[m]*(1 << lev);
+    std::ostringstream fname;
+    fname << "VacuumCoolingT" << Dynamics::Time();
+    Cooling::ContinueCoolingSlot(fname.str(), StandardCoolingMaxSteps() + CalibrationCoolingMaxSteps(),
+                                 CalibrationBlockingLevel(), d, TCYM_U_OLD, a);
+    DeltaNCsCalibration() = DeltaNCsCooling();
+    DeltaNCsRealTimeCalibration() = DeltaNCsCoolRealTime();
 }
 }  // namespace CoolingMethod
 }  // namespace ChernSimonsNumber

@@ -1,9 +1,13 @@
 // Exercises the reference-named entry points of include/tcym_dropin.hpp the way
 // a TopologicalCYMSolver driver calls them, on fields read from binary files.
-// usage: dropin_check nc N U.bin E.bin out_prefix
+// usage: dropin_check nc N U.bin E.bin out_prefix log_dir
 // writes out_prefix_U.bin / out_prefix_E.bin after 4 Dynamics::Update steps
 // (dTau 0.05), then 4 more steps inside a Resident scope into *_U2/_E2, and
-// prints "ncsdot T00 gauss_max unitarity_max" for the first state.
+// prints "ncsdot T00 gauss_max unitarity_max TXX TZZ LambdaXX LambdaZZ" for
+// the first state.  Then cools a blocked copy into *_C (printing its extent,
+// spacing, blocking number and flow clock) and calibrates the cooling method
+// (printing DeltaNCsCalibration, blocking number and flow clock; the log goes
+// to log_dir).
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -23,7 +27,7 @@ static void wr(const std::string& f, const double* p, size_t n) {
 }
 
 int main(int argc, char** argv) {
-    if (argc != 6) { fprintf(stderr, "usage: dropin_check nc N U.bin E.bin out_prefix\n"); return 2; }
+    if (argc != 7) { fprintf(stderr, "usage: dropin_check nc N U.bin E.bin out_prefix log_dir\n"); return 2; }
     const int nc = atoi(argv[1]), N = atoi(argv[2]);
     tcym_dropin::Configure(nc, TCYM_FP64, 0);
     GaugeLinks U(N, N, N, 1.0, 1.0, 1.0);
@@ -51,6 +55,27 @@ int main(int argc, char** argv) {
         }
         wr(std::string(argv[5]) + "_U2.bin", U.U, nu);
         wr(std::string(argv[5]) + "_E2.bin", E.E, ne);
+
+        // a cold copy blocked once after flow step 3 of 8
+        GradientFlow::SqrtDcTauOverSpacing() = 0.2;
+        Cooling::BlockFrequency() = 3;
+        GaugeLinks UC(N, N, N, 1.0, 1.0, 1.0);
+        Cooling::CoolNSave("cold", 8, 1, &U, &UC);
+        wr(std::string(argv[5]) + "_C.bin", UC.U, (size_t)3*tcym_dropin::LinkSize()*UC.Volume);
+        printf("%d %.17g %d %d %.17g\n", UC.N[0], UC.a[0], Cooling::BlockingNumber(), GradientFlow::CoolingSteps(),
+               GradientFlow::CoolingTime());
+        // CoolingMethod::Start with 4 flow steps, then Calibrate: 5 more calibration
+        // steps, blocked after step 6, logged to <dir>/VacuumCoolingT0.4ID7.log
+        ChernSimonsNumber::CoolingMethod::StandardCoolingMaxSteps() = 4;
+        ChernSimonsNumber::CoolingMethod::CalibrationCoolingMaxSteps() = 5;
+        ChernSimonsNumber::CoolingMethod::CalibrationBlockingLevel() = 1;
+        ChernSimonsNumber::CoolingMethod::Start(&U);
+        Cooling::BlockFrequency() = 6;
+        IO::SetOutputDirectory(argv[6]);
+        RandomNumberGenerator::MySEED() = 7;
+        ChernSimonsNumber::CoolingMethod::Calibrate(&U);
+        printf("%.17g %d %d %.17g\n", ChernSimonsNumber::DeltaNCsCalibration(), Cooling::BlockingNumber(),
+               GradientFlow::CoolingSteps(), GradientFlow::CoolingTime());
     } catch (const tcym_dropin::Error& e) {
         fprintf(stderr, "%s (code %d)\n", e.what(), e.code);
         return 1;

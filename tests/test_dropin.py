@@ -1,7 +1,7 @@
 """The C++ drop-in (include/tcym_dropin.hpp): a program written against the
 reference's Dynamics::Update / NCsDot / Bulk / HardScales / GaussLaw /
-Unitarity entry points over GaugeLinks/ElectricFields runs on the GPU and
-matches the oracle."""
+Unitarity / Cooling::CoolNSave / CoolingMethod::Calibrate entry points over
+GaugeLinks/ElectricFields runs on the GPU and matches the oracle."""
 import os
 import subprocess
 
@@ -26,10 +26,11 @@ def test_dropin_matches_oracle(nc, tmp_path):
     lat.random(2024, 0.4, 0.3, restore=True)
     lat.U.tofile(tmp_path / "U.bin")
     lat.E.tofile(tmp_path / "E.bin")
-    r = subprocess.run([EXE, str(nc), str(N), str(tmp_path / "U.bin"), str(tmp_path / "E.bin"), str(tmp_path / "out")],
-                       capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE, str(nc), str(N), str(tmp_path / "U.bin"), str(tmp_path / "E.bin"), str(tmp_path / "out"),
+                        str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    ncs, t00, gmax, umax, txx, tzz, lxx, lzz = map(float, r.stdout.split())
+    lines = r.stdout.splitlines()
+    ncs, t00, gmax, umax, txx, tzz, lxx, lzz = map(float, lines[0].split())
     lat.leapfrog(4, 0.05)
     np.testing.assert_allclose(np.fromfile(tmp_path / "out_U.bin"), lat.U, atol=1e-12)
     np.testing.assert_allclose(np.fromfile(tmp_path / "out_E.bin"), lat.E, atol=1e-12)
@@ -47,3 +48,45 @@ def test_dropin_matches_oracle(nc, tmp_path):
     lat.leapfrog(4, 0.05)
     np.testing.assert_allclose(np.fromfile(tmp_path / "out_U2.bin"), lat.U, atol=1e-12)
     np.testing.assert_allclose(np.fromfile(tmp_path / "out_E2.bin"), lat.E, atol=1e-12)
+
+    # Cooling::CoolNSave(8 steps, MaxBlockingNumber 1) with BlockFrequency 3:
+    # blocked after step 3, the cold copy is the 3^3 lattice at spacing 2
+    dtau = 0.2 * 0.2
+    cold, nb = lat.copy(), 0
+    for k in range(1, 9):
+        cold.flow(1, dtau)
+        if k % (3 * 8 ** nb) == 0 and nb < 1:
+            cold = cold.blocked()
+            nb += 1
+    n_c, a_c, nb_c, steps_c, ctau_c = lines[1].split()
+    assert (int(n_c), float(a_c), int(nb_c), int(steps_c)) == (N // 2, 2.0, 1, 8)
+    # 3 steps at a = 1 and 5 at a = 2: (a^3)^(2/3) = 1 and 4
+    assert abs(float(ctau_c) - dtau * (3 + 5 * 4)) < 1e-14
+    np.testing.assert_allclose(np.fromfile(tmp_path / "out_C.bin"), cold.U, atol=1e-12)
+    # CoolingMethod::Start (4 steps) then Calibrate: calibration steps 5..9 from
+    # the cooled copy with E zero, blocked after step 6 (BlockFrequency 6)
+    work = lat.copy()
+    work.flow(4, dtau)
+    work.E[:] = 0.0
+    acc, ctau, nb, rows = 0.0, 4 * dtau, 0, []
+
+    def row(w):
+        b = w.bulk()
+        cb = (w.p.a[0] / w.p.a[0] ** 3) ** 2
+        return [ctau, 0.5 * cb * b[3:].sum(), 0.5 * cb * b[:3].sum(), acc, w.vacuum_ncs()]
+
+    rows.append(row(work))
+    for k in range(5, 10):
+        acc += work.flow_calibration(1, dtau)
+        ctau += dtau * (work.p.a[0] ** 3) ** (2.0 / 3.0)
+        rows.append(row(work))
+        if k % (6 * 8 ** nb) == 0 and nb < 1:
+            work = work.blocked()
+            nb += 1
+    d_cal, nb_cal, steps_cal, ctau_cal = lines[2].split()
+    assert (int(nb_cal), int(steps_cal)) == (1, 9)
+    assert abs(float(d_cal) - acc) <= 1e-10 * max(abs(acc), 1e-12)
+    assert abs(float(ctau_cal) - ctau) < 1e-13
+    # the log file, written with the stream's default 6 significant digits
+    log = np.loadtxt(tmp_path / "VacuumCoolingT0.4ID7.log")
+    np.testing.assert_allclose(log, np.array(rows), rtol=1e-5, atol=1e-9)
