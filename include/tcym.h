@@ -41,6 +41,7 @@ extern "C" {
 #define TCYM_E_NOCONV        -3   /* Gauss-law restoration hit its iteration cap */
 #define TCYM_E_NONFINITE     -4   /* a reduction produced NaN/Inf */
 #define TCYM_E_UNSUPPORTED   -5   /* device is not gfx950, or feature not built */
+#define TCYM_E_IO            -6   /* a file could not be opened, read or written, or is malformed (tcym_io.h) */
 
 #define TCYM_FP64 0               /* FP64 storage and arithmetic */
 #define TCYM_FP32 1               /* FP32 storage/arithmetic; FP64 reunitarisation,
@@ -77,6 +78,9 @@ typedef struct {
 int tcym_create(const tcym_params *p, tcym_ctx **out);      /* Lattice::Init (LATTICE/GLinksEFields.cpp:36-45) */
 int tcym_destroy(tcym_ctx *ctx);
 const char *tcym_last_error(const tcym_ctx *ctx);
+/* the parameters the context was created with (no reference counterpart:
+   the reference reads the Lattice namespace globals) */
+int tcym_get_params(const tcym_ctx *ctx, tcym_params *p);
 const char *tcym_version(void);
 
 /* ---- fields ---- */

@@ -13,6 +13,7 @@
 //   ImprovedOperators::ComputeEDotB                            CALC/ChernSimonsOperators.cpp:384
 //   Observables::{Bulk,GaussLaw,Unitarity}, EdotBMap::CreateMap OBSERVABLES/*.cpp
 //   GaussLawRestoration::Restore                               GAUSSLAWFIXING/GaussRestore.cpp:131
+//   IO::SaveConfiguration / LoadConfiguration                  IO/SaveConfiguration.cpp:3, IO/LoadConfiguration.cpp:3
 // includes this header instead of those sources and links libtcym.so.
 //
 // Group and precision are run-time values (tcym_dropin::Configure) instead of
@@ -33,9 +34,11 @@
 #include <stdexcept>
 #include <string>
 #include <fstream>
+#include <iostream>
 #include <sstream>
 
 #include "tcym.h"
+#include "tcym_io.h"
 
 typedef int INT;
 typedef double DOUBLE;
@@ -126,16 +129,26 @@ namespace tcym_dropin {
 class Device {
 public:
     static std::map<std::string, Device*>& registry() { static std::map<std::string, Device*> m; return m; }
+    static std::string key(const INT* N, const DOUBLE* a) {
+        char k[160];
+        snprintf(k, sizeof k, "%d %d %d %d %d %.17g %.17g %.17g %.17g", config().nc, config().precision,
+                 N[0], N[1], N[2], a[0], a[1], a[2], Lattice::aScale());
+        return k;
+    }
+    // the context of a lattice geometry, created on first use
     static Device& get(const INT* N, const DOUBLE* a) {
         std::map<std::string, Device*>& all = registry();
-        char key[160];
-        snprintf(key, sizeof key, "%d %d %d %d %d %.17g %.17g %.17g %.17g", config().nc, config().precision,
-                 N[0], N[1], N[2], a[0], a[1], a[2], Lattice::aScale());
-        auto it = all.find(key);
+        const std::string k = key(N, a);
+        auto it = all.find(k);
         if (it != all.end()) return *it->second;
         Device* d = new Device(N, a);
-        all[key] = d;
+        all[k] = d;
         return *d;
+    }
+    // the context of a lattice geometry if one exists
+    static Device* find(const INT* N, const DOUBLE* a) {
+        auto it = registry().find(key(N, a));
+        return it == registry().end() ? nullptr : it->second;
     }
     tcym_ctx* ctx = nullptr;
     void check(int r) const {
@@ -219,9 +232,9 @@ struct Resident {
     }
 };
 // download every device-newer field of the lattice's context to its host object
-inline void SyncToHost(GaugeLinks* U) { Device& d = Device::get(U->N, U->a); d.flush(); }
+inline void SyncToHost(GaugeLinks* U) { if (Device* d = Device::find(U->N, U->a)) d->flush(); }
 // host arrays were modified by the caller: re-upload on next use
-inline void HostModified(GaugeLinks* U) { Device::get(U->N, U->a).invalidate(); }
+inline void HostModified(GaugeLinks* U) { if (Device* d = Device::find(U->N, U->a)) d->invalidate(); }
 
 }  // namespace tcym_dropin
 
@@ -591,3 +604,47 @@ inline void Restore(GaugeLinks* U, ElectricFields* E) {   // GAUSSLAWFIXING/Gaus
     d.wrote_field(e);
 }
 }  // namespace GaussLawRestoration
+
+// ---------------------------------------------------------------- text configurations
+namespace IO {
+// IO::InputDirectory / InputUFile / InputEFile (IO/InputManagement.cpp:3-28)
+inline std::string& InputDirectory() { static std::string d; return d; }
+inline void SetInputDirectory(const std::string& x) { InputDirectory() = x + "/"; }
+inline std::string& InputUFile() { static std::string f; return f; }
+inline std::string& InputEFile() { static std::string f; return f; }
+template <class A, class B>
+inline void SetInputFile(A x, B y) {
+    std::ostringstream u, e;
+    u << "UOutT" << x << "ID" << y << ".txt";
+    e << "EOutT" << x << "ID" << y << ".txt";
+    InputUFile() = u.str();
+    InputEFile() = e.str();
+}
+inline void io_check(int r) {
+    if (r != TCYM_OK) throw tcym_dropin::Error(r, std::string("tcym: ") + tcym_io_last_error());
+}
+// IO::SaveConfiguration (IO/SaveConfiguration.cpp:3-52): OutputDirectory +
+// name + "ID" + MySEED + ".txt", 15 significant digits
+inline void SaveConfiguration(std::string Ufname, std::string Efname, GaugeLinks* U, ElectricFields* E) {
+    tcym_dropin::SyncToHost(U);
+    std::ostringstream uf, ef;
+    uf << OutputDirectory() << Ufname << "ID" << RandomNumberGenerator::MySEED() << ".txt";
+    ef << OutputDirectory() << Efname << "ID" << RandomNumberGenerator::MySEED() << ".txt";
+    const int nc = tcym_dropin::config().nc;
+    io_check(tcym_write_text_links(nc, U->N, U->U, uf.str().c_str(), 15, 0));
+    io_check(tcym_write_text_efield(nc, E->N, E->E, ef.str().c_str(), 15, 0));
+}
+// IO::LoadConfiguration (IO/LoadConfiguration.cpp:3-136): InputDirectory +
+// InputUFile / InputEFile into the host fields, then the initial energy
+// densities and pressures on stderr
+inline void LoadConfiguration(GaugeLinks* U, ElectricFields* E) {
+    const int nc = tcym_dropin::config().nc;
+    io_check(tcym_read_text_links(nc, U->N, U->U, (InputDirectory() + InputUFile()).c_str(), 0));
+    io_check(tcym_read_text_efield(nc, E->N, E->E, (InputDirectory() + InputEFile()).c_str(), 0));
+    tcym_dropin::HostModified(U);
+    Observables::Bulk::Update(U, E);
+    std::cerr << "#INITIAL ENERGY DENSITIES AND PRESSURES -- T00 TXX TYY TZZ" << std::endl;
+    std::cerr << Observables::Bulk::T00() << " " << Observables::Bulk::TXX() << " " << Observables::Bulk::TYY() << " "
+              << Observables::Bulk::TZZ() << std::endl;
+}
+}  // namespace IO

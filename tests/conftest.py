@@ -10,5 +10,8 @@ if TESTS not in sys.path:
 
 
 def pytest_configure(config):
+    # temporary files of tests stay inside the repository (a git-ignored folder)
+    if not config.option.basetemp:
+        config.option.basetemp = os.path.join(ROOT, ".pytest_tmp")
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
     config.addinivalue_line("markers", "slow: longer CPU-side checks")

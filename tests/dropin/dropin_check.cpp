@@ -7,7 +7,8 @@
 // the first state.  Then cools a blocked copy into *_C (printing its extent,
 // spacing, blocking number and flow clock) and calibrates the cooling method
 // (printing DeltaNCsCalibration, blocking number and flow clock; the log goes
-// to log_dir).
+// to log_dir), saves the hot fields as the reference's text files in log_dir,
+// loads them into new containers and prints their NCsDot.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -76,6 +77,15 @@ int main(int argc, char** argv) {
         ChernSimonsNumber::CoolingMethod::Calibrate(&U);
         printf("%.17g %d %d %.17g\n", ChernSimonsNumber::DeltaNCsCalibration(), Cooling::BlockingNumber(),
                GradientFlow::CoolingSteps(), GradientFlow::CoolingTime());
+        // the hot fields as text (<dir>/UOutT0.4ID7.txt, EOutT0.4ID7.txt), read
+        // back into new containers
+        IO::SaveConfiguration("UOutT0.4", "EOutT0.4", &U, &E);
+        IO::SetInputDirectory(argv[6]);
+        IO::SetInputFile(0.4, 7);
+        GaugeLinks U3(N, N, N, 1.0, 1.0, 1.0);
+        ElectricFields E3(N, N, N, 1.0, 1.0, 1.0);
+        IO::LoadConfiguration(&U3, &E3);
+        printf("%.17g\n", ChernSimonsNumber::NCsDot(&U3, &E3));
     } catch (const tcym_dropin::Error& e) {
         fprintf(stderr, "%s (code %d)\n", e.what(), e.code);
         return 1;

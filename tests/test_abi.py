@@ -1,6 +1,6 @@
 """CPU-side checks of the drop-in boundary: the C-ABI library loads and exports
-every entry point that include/tcym.h and include/tcym_batch.h declare (no
-compute calls here)."""
+every entry point that include/tcym.h, include/tcym_batch.h and
+include/tcym_io.h declare (no compute calls here)."""
 import os
 import re
 
@@ -26,6 +26,7 @@ def all_declared():
 def test_header_declares_what_the_wrapper_binds():
     assert declared_symbols("tcym.h") == sorted(T.EXPORTS)
     assert declared_symbols("tcym_batch.h") == sorted(T.BATCH_EXPORTS)
+    assert declared_symbols("tcym_io.h") == sorted(T.IO_EXPORTS)
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,6 +1,7 @@
 """The C++ drop-in (include/tcym_dropin.hpp): a program written against the
 reference's Dynamics::Update / NCsDot / Bulk / HardScales / GaussLaw /
-Unitarity / Cooling::CoolNSave / CoolingMethod::Calibrate entry points over
+Unitarity / Cooling::CoolNSave / CoolingMethod::Calibrate /
+IO::SaveConfiguration / IO::LoadConfiguration entry points over
 GaugeLinks/ElectricFields runs on the GPU and matches the oracle."""
 import os
 import subprocess
@@ -9,6 +10,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
+import topologicalcymsolver_amd as T
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "tests", "dropin", "_build", "dropin_check")
@@ -90,3 +92,12 @@ def test_dropin_matches_oracle(nc, tmp_path):
     # the log file, written with the stream's default 6 significant digits
     log = np.loadtxt(tmp_path / "VacuumCoolingT0.4ID7.log")
     np.testing.assert_allclose(log, np.array(rows), rtol=1e-5, atol=1e-9)
+    # IO::SaveConfiguration with the reference's 15 digits, IO::LoadConfiguration back
+    U2, E2 = np.fromfile(tmp_path / "out_U2.bin"), np.fromfile(tmp_path / "out_E2.bin")
+    U15 = np.array([float("%.15g" % v) for v in U2])
+    E15 = np.array([float("%.15g" % v) for v in E2])
+    assert np.array_equal(T.read_text("links", nc, (N, N, N), tmp_path / "UOutT0.4ID7.txt"), U15)
+    assert np.array_equal(T.read_text("efield", nc, (N, N, N), tmp_path / "EOutT0.4ID7.txt"), E15)
+    back = lat.copy()
+    back.U[:], back.E[:] = U15, E15
+    assert abs(float(lines[3]) - back.ncsdot()) < 1e-13

@@ -16,7 +16,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TCYM_LIBRARY") or os.path.join(PKG, "libtcym.so")
 
-OK, E_ARG, E_HIP, E_NOCONV, E_NONFINITE, E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
+OK, E_ARG, E_HIP, E_NOCONV, E_NONFINITE, E_UNSUPPORTED, E_IO = 0, -1, -2, -3, -4, -5, -6
 FP64, FP32 = 0, 1
 U_HOT, U_COOL, U_OLD, U_NEW, U_MID, U_AUX = range(6)
 E_HOT, E_COOL, E_MID, E_XI, E_AUX = range(5)
@@ -33,6 +33,13 @@ EXPORTS = [
     "tcym_gauss_restore", "tcym_geodesic", "tcym_delta_ncs_cooling", "tcym_vacuum_ncs",
     "tcym_block", "tcym_level", "tcym_cool_blocked", "tcym_continue_cooling",
     "tcym_timing", "tcym_kernel_time", "tcym_timing_reset", "tcym_sync", "tcym_device_bytes",
+    "tcym_get_params",
+]
+# every exported symbol of include/tcym_io.h
+IO_EXPORTS = [
+    "tcym_io_last_error", "tcym_write_text_links", "tcym_write_text_efield", "tcym_read_text_links",
+    "tcym_read_text_efield", "tcym_save_text", "tcym_load_text", "tcym_write_checkpoint",
+    "tcym_read_checkpoint", "tcym_save_checkpoint", "tcym_load_checkpoint",
 ]
 # every exported symbol of include/tcym_batch.h
 BATCH_EXPORTS = [
@@ -61,6 +68,19 @@ class BatchParams(ctypes.Structure):
     _fields_ = [("nc", ctypes.c_int), ("precision", ctypes.c_int), ("count", ctypes.c_int),
                 ("N", ctypes.POINTER(ctypes.c_int)), ("a", ctypes.c_double * 3), ("aScale", ctypes.c_double),
                 ("device", ctypes.c_int)]
+
+
+class CheckpointInfo(ctypes.Structure):
+    """tcym_checkpoint_info (include/tcym_io.h)."""
+    _fields_ = [("nc", ctypes.c_int), ("precision", ctypes.c_int), ("N", ctypes.c_int * 3),
+                ("a", ctypes.c_double * 3), ("aScale", ctypes.c_double), ("step", ctypes.c_uint64),
+                ("seed", ctypes.c_uint64), ("time", ctypes.c_double), ("acc", ctypes.c_double * 8),
+                ("has_efield", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"nc": self.nc, "precision": self.precision, "N": tuple(self.N), "a": tuple(self.a),
+                "aScale": self.aScale, "step": self.step, "seed": self.seed, "time": self.time,
+                "acc": tuple(self.acc), "has_efield": bool(self.has_efield)}
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -113,6 +133,17 @@ def load_library(path=LIB_PATH):
         "tcym_batch_gauss": [V, I, I, _dp], "tcym_batch_unitarity": [V, I, _dp],
         "tcym_batch_tune": [V, I, D, I, ctypes.POINTER(I), _dp], "tcym_batch_timing": [V, I],
         "tcym_batch_kernel_time": [V, ctypes.c_char_p, _dp, LL], "tcym_batch_timing_reset": [V],
+        "tcym_get_params": [V, P],
+        "tcym_write_text_links": [I, ctypes.POINTER(I), _dp, ctypes.c_char_p, I, I],
+        "tcym_write_text_efield": [I, ctypes.POINTER(I), _dp, ctypes.c_char_p, I, I],
+        "tcym_read_text_links": [I, ctypes.POINTER(I), _dp, ctypes.c_char_p, I],
+        "tcym_read_text_efield": [I, ctypes.POINTER(I), _dp, ctypes.c_char_p, I],
+        "tcym_save_text": [V, I, I, ctypes.c_char_p, ctypes.c_char_p, I, I],
+        "tcym_load_text": [V, I, I, ctypes.c_char_p, ctypes.c_char_p, I],
+        "tcym_write_checkpoint": [ctypes.c_char_p, ctypes.POINTER(CheckpointInfo), _dp, _dp],
+        "tcym_read_checkpoint": [ctypes.c_char_p, ctypes.POINTER(CheckpointInfo), _dp, _dp],
+        "tcym_save_checkpoint": [V, I, I, ctypes.c_char_p, ctypes.POINTER(CheckpointInfo)],
+        "tcym_load_checkpoint": [V, I, I, ctypes.c_char_p, ctypes.POINTER(CheckpointInfo)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -124,8 +155,76 @@ def load_library(path=LIB_PATH):
     L.tcym_version.restype = ctypes.c_char_p
     L.tcym_batch_last_error.argtypes = [V]
     L.tcym_batch_last_error.restype = ctypes.c_char_p
+    L.tcym_io_last_error.argtypes = []
+    L.tcym_io_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
+
+
+# ---- configuration files (include/tcym_io.h); host arrays, no device needed ----
+def _io_ck(r):
+    if r != OK:
+        raise TcymError(r, load_library().tcym_io_last_error().decode())
+
+
+def _record_size(nc, kind):
+    if kind not in ("links", "efield"):
+        raise ValueError("kind must be 'links' or 'efield'")
+    return (4 if nc == 2 else 18) if kind == "links" else (3 if nc == 2 else 8)
+
+
+def write_text(kind, nc, N, A, path, digits=15, nthreads=0):
+    """IO::SaveConfiguration's text file of links or E (IO/SaveConfiguration.cpp:15-51);
+    digits 15 is the reference's OUTPUT_PRECISION, 17 reads back bit-exact."""
+    L = load_library()
+    n = (ctypes.c_int * 3)(*N)
+    A = _arr(A, 3 * N[0] * N[1] * N[2] * _record_size(nc, kind))
+    fn = L.tcym_write_text_links if kind == "links" else L.tcym_write_text_efield
+    _io_ck(fn(nc, n, A.ctypes.data_as(_dp), os.fsencode(path), digits, nthreads))
+
+
+def read_text(kind, nc, N, path, nthreads=0):
+    """IO::LoadConfiguration of one text file (IO/LoadConfiguration.cpp:46-122)."""
+    L = load_library()
+    n = (ctypes.c_int * 3)(*N)
+    A = np.empty(3 * N[0] * N[1] * N[2] * _record_size(nc, kind))
+    fn = L.tcym_read_text_links if kind == "links" else L.tcym_read_text_efield
+    _io_ck(fn(nc, n, A.ctypes.data_as(_dp), os.fsencode(path), nthreads))
+    return A
+
+
+def _info(nc=2, precision=FP64, N=(1, 1, 1), a=(1.0, 1.0, 1.0), aScale=1.0, step=0, seed=0, time=0.0, acc=()):
+    acc = tuple(acc) + (0.0,) * (8 - len(acc))
+    return CheckpointInfo(nc, precision, (ctypes.c_int * 3)(*N), (ctypes.c_double * 3)(*a), aScale, step, seed,
+                          time, (ctypes.c_double * 8)(*acc))
+
+
+def write_checkpoint(path, U, E=None, **info):
+    """Binary checkpoint of host arrays; info: nc, precision, N, a, aScale, step, seed, time, acc."""
+    ci = _info(**info)
+    su, ne = (4, 3) if ci.nc == 2 else (18, 8)
+    V = ci.N[0] * ci.N[1] * ci.N[2]
+    U = _arr(U, 3 * V * su)
+    E = None if E is None else _arr(E, 3 * V * ne)
+    _io_ck(load_library().tcym_write_checkpoint(os.fsencode(path), ctypes.byref(ci), U.ctypes.data_as(_dp),
+                                                None if E is None else E.ctypes.data_as(_dp)))
+
+
+def read_checkpoint(path, arrays=True):
+    """(info dict, U, E) of a binary checkpoint; E is None for a links-only file,
+    U and E are None when arrays is False."""
+    L = load_library()
+    ci = CheckpointInfo()
+    _io_ck(L.tcym_read_checkpoint(os.fsencode(path), ctypes.byref(ci), None, None))
+    if not arrays:
+        return ci.as_dict(), None, None
+    su, ne = (4, 3) if ci.nc == 2 else (18, 8)
+    V = ci.N[0] * ci.N[1] * ci.N[2]
+    U = np.empty(3 * V * su)
+    E = np.empty(3 * V * ne) if ci.has_efield else None
+    _io_ck(L.tcym_read_checkpoint(os.fsencode(path), ctypes.byref(ci), U.ctypes.data_as(_dp),
+                                  None if E is None else E.ctypes.data_as(_dp)))
+    return ci.as_dict(), U, E
 
 
 def _arr(a, n=None):
@@ -355,6 +454,37 @@ class Solver:
         return d.value
 
     # ---- measurement ----
+    # ---- configuration files (include/tcym_io.h) ----
+    def _io(self, r):
+        if r != OK:
+            raise TcymError(r, self.L.tcym_io_last_error().decode())
+
+    def save_configuration(self, ufile, efile, u=U_HOT, e=E_HOT, digits=15, nthreads=0):
+        """IO::SaveConfiguration (IO/SaveConfiguration.cpp:3-52); either file may be None."""
+        self._io(self.L.tcym_save_text(self.h, u, e, None if ufile is None else os.fsencode(ufile),
+                                       None if efile is None else os.fsencode(efile), digits, nthreads))
+
+    def load_configuration(self, ufile, efile, u=U_HOT, e=E_HOT, nthreads=0):
+        """IO::LoadConfiguration (IO/LoadConfiguration.cpp:3-136); either file may be None."""
+        self._io(self.L.tcym_load_text(self.h, u, e, None if ufile is None else os.fsencode(ufile),
+                                       None if efile is None else os.fsencode(efile), nthreads))
+
+    def save_checkpoint(self, path, u=U_HOT, e=E_HOT, step=0, seed=0, time=0.0, acc=()):
+        ci = _info(step=step, seed=seed, time=time, acc=acc)
+        self._io(self.L.tcym_save_checkpoint(self.h, u, e, os.fsencode(path), ctypes.byref(ci)))
+
+    def load_checkpoint(self, path, u=U_HOT, e=E_HOT):
+        """Loads a checkpoint into the slots; returns its header as a dict."""
+        ci = CheckpointInfo()
+        self._io(self.L.tcym_load_checkpoint(self.h, u, e, os.fsencode(path), ctypes.byref(ci)))
+        return ci.as_dict()
+
+    def params(self):
+        p = Params()
+        self._ck(self.L.tcym_get_params(self.h, ctypes.byref(p)))
+        return {"nc": p.nc, "precision": p.precision, "N": tuple(p.N), "a": tuple(p.a), "aScale": p.aScale,
+                "device": p.device}
+
     def timing(self, enable=True):
         self._ck(self.L.tcym_timing(self.h, 1 if enable else 0))
 

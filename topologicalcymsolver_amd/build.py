@@ -2,8 +2,9 @@
 
     python -m topologicalcymsolver_amd.build        # or build() from Python
 
-Each .hip translation unit is compiled with hipcc for gfx950 only, in
-parallel, and linked into topologicalcymsolver_amd/libtcym.so.
+Each .hip translation unit is compiled with hipcc for gfx950 only, the
+host-only C++ sources (configuration files) with the host C++ compiler, in
+parallel, and all are linked into topologicalcymsolver_amd/libtcym.so.
 """
 import concurrent.futures as cf
 import os
@@ -18,9 +19,12 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libtcym.so")
 SOURCES = ["tcym_api.hip", "tcym_batch.hip", "tcym_kernels_su2.hip", "tcym_kernels_su3.hip"]
+HOST_SOURCES = ["tcym_io.cpp"]
 DEPS = ["tcym_backend.hpp", "tcym_device.hpp", "tcym_group.hpp", "tcym_kernels.inc", "tcym_reduce.hpp"]
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", shutil.which("g++") or "c++")
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + CSRC]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions stay disabled; FP32 code is scalar VALU
@@ -31,10 +35,13 @@ FLAGS = [
 
 def _compile(src, build_dir=None, extra=()):
     obj = os.path.join(build_dir or BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + list(extra) + ["-c", os.path.join(CSRC, src), "-o", obj]
+    if src in HOST_SOURCES:
+        cmd = [CXX] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [HIPCC] + FLAGS + list(extra) + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, r.stdout, r.stderr))
+        raise RuntimeError("%s failed for %s:\n%s\n%s" % (cmd[0], src, r.stdout, r.stderr))
     return obj
 
 
@@ -44,9 +51,9 @@ def build(verbose=False, jobs=None, out=None, defines=()):
     lib = out or LIB
     bdir = os.path.join(os.path.dirname(lib), "_build") if out else BUILD
     os.makedirs(bdir, exist_ok=True)
-    jobs = jobs or min(len(SOURCES), os.cpu_count() or 1)
+    jobs = jobs or min(len(SOURCES) + len(HOST_SOURCES), os.cpu_count() or 1)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, bdir, defines), SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, bdir, defines), SOURCES + HOST_SOURCES))
     tmp = lib + ".tmp"
     r = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs,
                        capture_output=True, text=True)

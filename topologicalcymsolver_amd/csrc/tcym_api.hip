@@ -199,6 +199,14 @@ int tcym_create(const tcym_params* p, tcym_ctx** out) {
     return TCYM_OK;
 }
 
+int tcym_get_params(const tcym_ctx* c, tcym_params* p) {
+    if (!c || !p) return TCYM_E_ARG;
+    p->nc = c->nc; p->precision = c->prec; p->device = c->device;
+    for (int k = 0; k < 3; k++) { p->N[k] = c->g.n[k]; p->a[k] = c->g.a[k]; }
+    p->aScale = c->g.aScale;
+    return TCYM_OK;
+}
+
 int tcym_destroy(tcym_ctx* c) {
     if (!c) return TCYM_OK;
     if (c->stream) hipStreamSynchronize(c->stream);
