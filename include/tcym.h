@@ -168,6 +168,18 @@ int tcym_cool_blocked(tcym_ctx *ctx, int u_hot, int u_cold, int nsteps, double d
 int tcym_continue_cooling(tcym_ctx *ctx, int u_src, int u_work, int e_work, int step0, int nsteps, double dtau,
                           double ctau0, int block_frequency, int max_blocking, int *nblocked, double *dncs,
                           double *log);
+/* ---- thermal initial condition ---- */
+/* ThermalDynamics::Thermalize(MaxSteps = cycles, MaxTime) (DYNAMICS/ThermalDynamics.cpp:260-316):
+   per cycle i, E^a ~ N(0, 1/beta) (Philox stream 3, counter step i, replacing
+   RandomNumberGenerator::Gauss()/sqrt(beta), :205-226) and Gauss-law
+   restoration (maxiter, tol as tcym_gauss_restore), then leapfrog steps of dt
+   (the reference's dTau = 0.0375) while steps*dt < max_time.  Every 20 steps a
+   row {Time() + i max_time, T00, TXX, TYY, TZZ, ELECTRIC, MAGNETIC} of the
+   reference's ThermalEnergy file goes to log (log_rows rows of 7 doubles at
+   most; *nrows receives the number of rows produced).  The links are evolved
+   from their current values (the reference starts from InitialConditions::SetZero). */
+int tcym_thermalize(tcym_ctx *ctx, int u, int e, int cycles, double max_time, double dt, double beta, uint64_t seed,
+                    int maxiter, double tol, double *log, int log_rows, int *nrows);
 /* FP64 reunitarisation of every link (no reference counterpart: the FP32 form applies it after each step) */
 int tcym_reunitarize(tcym_ctx *ctx, int u);
 

@@ -14,6 +14,7 @@
 //   Observables::{Bulk,GaussLaw,Unitarity}, EdotBMap::CreateMap OBSERVABLES/*.cpp
 //   GaussLawRestoration::Restore                               GAUSSLAWFIXING/GaussRestore.cpp:131
 //   IO::SaveConfiguration / LoadConfiguration                  IO/SaveConfiguration.cpp:3, IO/LoadConfiguration.cpp:3
+//   InitialConditions::SetZero, ThermalDynamics::Thermalize    INITIALCONDITIONS/SetZero.cpp:3, DYNAMICS/ThermalDynamics.cpp:260
 // includes this header instead of those sources and links libtcym.so.
 //
 // Group and precision are run-time values (tcym_dropin::Configure) instead of
@@ -266,9 +267,14 @@ inline void Update(GaugeLinks* U, ElectricFields* E) {   // DYNAMICS/Dynamics.cp
 }
 }  // namespace Dynamics
 
+namespace ThermalDynamics {
+inline DOUBLE& beta() { static DOUBLE v = 2.0; return v; }      // DYNAMICS/ThermalDynamics.cpp:10
+inline DOUBLE& dTau() { static DOUBLE v = 0.0375; return v; }   // :23
+}
+
 namespace LangevinDynamics {
 inline DOUBLE& dTauL() { static DOUBLE v = 0.001; return v; }   // DYNAMICS/LangevinDynamics.cpp:41
-inline DOUBLE& beta() { static DOUBLE v = 2.0; return v; }      // ThermalDynamics::beta (DYNAMICS/ThermalDynamics.cpp:10)
+inline DOUBLE& beta() { return ThermalDynamics::beta(); }       // the Langevin temperature is ThermalDynamics::beta
 inline DOUBLE& damping() { static DOUBLE v = 1.0; return v; }   // fixed at 1 in the reference
 inline INT& tauLSteps() { static INT v = 0; return v; }
 inline uint64_t& seed() { static uint64_t v = 1; return v; }
@@ -648,3 +654,47 @@ inline void LoadConfiguration(GaugeLinks* U, ElectricFields* E) {
               << Observables::Bulk::TZZ() << std::endl;
 }
 }  // namespace IO
+
+// ---------------------------------------------------------------- initial conditions
+namespace InitialConditions {
+// InitialConditions::SetZero (INITIALCONDITIONS/SetZero.cpp:3-24): unit links, zero E
+inline void SetZero(GaugeLinks* U, ElectricFields* E) {
+    auto& d = tcym_dropin::dev(U);
+    const int u = d.links(U, TCYM_U_HOT), e = d.field(E, TCYM_E_HOT);
+    d.check(tcym_unit_links(d.ctx, u));
+    d.check(tcym_zero_efield(d.ctx, e));
+    d.wrote_links(u);
+    d.wrote_field(e);
+}
+}  // namespace InitialConditions
+
+namespace ThermalDynamics {
+// ThermalDynamics::Thermalize(MaxSteps, MaxTime) on the given fields
+// (DYNAMICS/ThermalDynamics.cpp:260-316): the ThermalEnergy rows go to
+// OutputDirectory + "ThermalEnergyID" + MySEED + ".txt"; the thermal E draws
+// use MySEED as their Philox key
+inline void Thermalize(INT MaxSteps, DOUBLE MaxTime, GaugeLinks* U, ElectricFields* E) {
+    auto& d = tcym_dropin::dev(U);
+    const int u = d.links(U, TCYM_U_HOT), e = d.field(E, TCYM_E_HOT);
+    const int cap = MaxSteps*((int)(MaxTime/dTau())/20 + 2);
+    double* log = new double[7*(size_t)(cap > 0 ? cap : 1)];
+    int n = 0;
+    const int r = tcym_thermalize(d.ctx, u, e, MaxSteps, MaxTime, dTau(), beta(),
+                                  (uint64_t)RandomNumberGenerator::MySEED(), 10000, 1e-14, log, cap, &n);
+    if (r != TCYM_OK) { delete[] log; d.check(r); }
+    std::ostringstream name;
+    name << IO::OutputDirectory() << "ThermalEnergyID" << RandomNumberGenerator::MySEED() << ".txt";
+    std::ofstream out(name.str().c_str());
+    for (int k = 0; k < n && k < cap; k++) {
+        const double* w = log + 7*k;
+        out << w[0] << " " << w[1] << " " << w[2] << " " << w[3] << " " << w[4] << " " << w[5] << " " << w[6] << std::endl;
+    }
+    delete[] log;
+    d.wrote_links(u);
+    d.wrote_field(e);
+    Observables::Bulk::Update(U, E);
+    std::cerr << "#INITIAL ENERGY DENSITIES AND PRESSURES -- T00 TXX TYY TZZ" << std::endl;
+    std::cerr << Observables::Bulk::T00() << " " << Observables::Bulk::TXX() << " " << Observables::Bulk::TYY() << " "
+              << Observables::Bulk::TZZ() << std::endl;
+}
+}  // namespace ThermalDynamics

@@ -138,6 +138,22 @@ void or_noise_field(const or_params *p, uint64_t seed, uint32_t step, double std
     }
 }
 
+/* ThermalDynamics::ReassignElectricFields: E^a ~ N(0, 1/beta), stream 3 with
+   the cycle as counter step (DYNAMICS/ThermalDynamics.cpp:205-221; Philox in
+   place of RandomNumberGenerator::Gauss) */
+void or_thermal_efield(const or_params *p, uint64_t seed, uint32_t cycle, double beta, double *E) {
+    int ne = or_ne(p->nc);
+    long L = 3*vol(p);
+    const double sigma = 1.0/sqrt(beta);
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (long l = 0; l < L; l++) {
+        double z[8];
+        or_link_normals(seed, 3u, cycle, (uint32_t)l, ne, z);
+        for (int a = 0; a < ne; a++) E[(size_t)ne*l + a] = sigma*z[a];
+    }
+}
+
 /* ---- lattice entry points ---- */
 void or_leapfrog(const or_params *p, double *U, double *E, int n, double dt) {
     DISPATCH(p, su2l_leapfrog(p, U, E, n, dt), su3l_leapfrog(p, U, E, n, dt));
@@ -233,4 +249,48 @@ double or_vacuum_ncs(const or_params *p, const double *U) {
 }
 void or_reunitarize_field(const or_params *p, double *U) {
     DISPATCH(p, su2l_reunitarize_field(p, U), su3l_reunitarize_field(p, U));
+}
+
+/* ThermalDynamics::Thermalize(cycles, max_time) (DYNAMICS/ThermalDynamics.cpp:260-316):
+   per cycle, ReassignElectricFields and GaussLawRestoration::Restore (:205-226),
+   then leapfrog steps while tSteps*dTau < MaxTime with a Bulk row
+   {Time() + i MaxTime, T00, TXX, TYY, TZZ, ELECTRIC, MAGNETIC} every 20 steps
+   (:274-292; BulkObservables.cpp:17-52, Minkowski); Reset (:85-91) after each
+   cycle.  Returns the number of rows (at most log_rows are stored). */
+int or_thermalize(const or_params *p, double *U, double *E, int cycles, double max_time, double dt, double beta,
+                  uint64_t seed, int maxiter, double tol, double *log, int log_rows) {
+    double cb[3];
+    for (int mu = 0; mu < 3; mu++) {
+        const double x = p->a[mu]*p->aScale*p->aScale/(p->a[0]*p->a[1]*p->a[2]);
+        cb[mu] = x*x;
+    }
+    int rows = 0;
+    for (int i = 0; i < cycles; i++) {
+        double fm;
+        or_thermal_efield(p, seed, (uint32_t)i, beta, E);
+        or_gauss_restore(p, U, E, maxiter, tol, &fm);
+        long t = 0;
+        while ((double)t*dt < max_time) {
+            or_leapfrog(p, U, E, 1, dt);
+            t++;
+            if (t % 20 == 0) {
+                double b[6];
+                or_bulk(p, U, E, b);
+                if (rows < log_rows) {
+                    double *w = log + 7*(size_t)rows;
+                    const double E0 = cb[0]*b[0], E1 = cb[1]*b[1], E2 = cb[2]*b[2];
+                    const double B0 = cb[0]*b[3], B1 = cb[1]*b[4], B2 = cb[2]*b[5];
+                    w[0] = (double)t*dt + i*max_time;
+                    w[1] = 0.5*(E0 + E1 + E2 + B0 + B1 + B2);
+                    w[2] = 0.5*((E1 + E2 + B1 + B2) - (E0 + B0));
+                    w[3] = 0.5*((E0 + E2 + B0 + B2) - (E1 + B1));
+                    w[4] = 0.5*((E0 + E1 + B0 + B1) - (E2 + B2));
+                    w[5] = 0.5*(E0 + E1 + E2);
+                    w[6] = 0.5*(B0 + B1 + B2);
+                }
+                rows++;
+            }
+        }
+    }
+    return rows;
 }

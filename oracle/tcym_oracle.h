@@ -58,6 +58,9 @@ void or_link_normals(uint64_t seed, uint32_t stream, uint32_t step, uint32_t lin
 void or_random_links(const or_params *p, uint64_t seed, double sigma, double *U);
 void or_random_efield(const or_params *p, uint64_t seed, double sigma, double *E);
 void or_noise_field(const or_params *p, uint64_t seed, uint32_t step, double stddev, double *Xi);
+void or_thermal_efield(const or_params *p, uint64_t seed, uint32_t cycle, double beta, double *E);
+int  or_thermalize(const or_params *p, double *U, double *E, int cycles, double max_time, double dt, double beta,
+                   uint64_t seed, int maxiter, double tol, double *log, int log_rows);
 
 /* ---- lattice sweeps ---- */
 void or_leapfrog(const or_params *p, double *U, double *E, int nsteps, double dt);

@@ -67,6 +67,8 @@ def lib():
             "or_geodesic": (None, [P, _dp, _dp, d, _dp, _dp]),
             "or_delta_ncs_cooling": (d, [P, _dp, _dp, _dp, _dp]),
             "or_vacuum_ncs": (d, [P, _dp]), "or_reunitarize_field": (None, [P, _dp]),
+            "or_thermal_efield": (None, [P, u64, u32, d, _dp]),
+            "or_thermalize": (i, [P, _dp, _dp, i, d, d, d, u64, i, d, _dp, i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -134,6 +136,15 @@ class Lattice:
         Xi = np.zeros_like(self.E)
         lib().or_noise_field(ctypes.byref(self.p), seed, step, stddev, _ptr(Xi))
         return Xi
+
+    def thermalize(self, cycles, max_time, dt, beta, seed, maxiter=10000, tol=1e-14):
+        """ThermalDynamics::Thermalize: returns the ThermalEnergy rows (n x 7)."""
+        cap = cycles * (int(max_time / dt) // 20 + 2)
+        log = np.zeros(7 * max(cap, 1))
+        n = lib().or_thermalize(ctypes.byref(self.p), _ptr(self.U), _ptr(self.E), cycles, max_time, dt, beta, seed,
+                                maxiter, tol, _ptr(log), cap)
+        assert n <= cap
+        return log[:7 * n].reshape(n, 7)
 
     def flow(self, n, dtau):
         lib().or_flow(ctypes.byref(self.p), _ptr(self.U), _ptr(self.E), n, dtau)

@@ -8,7 +8,8 @@
 // spacing, blocking number and flow clock) and calibrates the cooling method
 // (printing DeltaNCsCalibration, blocking number and flow clock; the log goes
 // to log_dir), saves the hot fields as the reference's text files in log_dir,
-// loads them into new containers and prints their NCsDot.
+// loads them into new containers and prints their NCsDot; finally thermalizes
+// a zero-field lattice into *_TU / *_TE (ThermalEnergy rows in log_dir).
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -86,6 +87,15 @@ int main(int argc, char** argv) {
         ElectricFields E3(N, N, N, 1.0, 1.0, 1.0);
         IO::LoadConfiguration(&U3, &E3);
         printf("%.17g\n", ChernSimonsNumber::NCsDot(&U3, &E3));
+        // the thermal initial condition: SetZero, then 2 cycles to time 1.6 at
+        // beta 4 (the rows go to <dir>/ThermalEnergyID7.txt)
+        GaugeLinks UT(N, N, N, 1.0, 1.0, 1.0);
+        ElectricFields ET(N, N, N, 1.0, 1.0, 1.0);
+        InitialConditions::SetZero(&UT, &ET);
+        ThermalDynamics::beta() = 4.0;
+        ThermalDynamics::Thermalize(2, 1.6, &UT, &ET);
+        wr(std::string(argv[5]) + "_TU.bin", UT.U, nu);
+        wr(std::string(argv[5]) + "_TE.bin", ET.E, ne);
     } catch (const tcym_dropin::Error& e) {
         fprintf(stderr, "%s (code %d)\n", e.what(), e.code);
         return 1;

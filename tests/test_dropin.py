@@ -1,7 +1,8 @@
 """The C++ drop-in (include/tcym_dropin.hpp): a program written against the
 reference's Dynamics::Update / NCsDot / Bulk / HardScales / GaussLaw /
 Unitarity / Cooling::CoolNSave / CoolingMethod::Calibrate /
-IO::SaveConfiguration / IO::LoadConfiguration entry points over
+IO::SaveConfiguration / IO::LoadConfiguration / ThermalDynamics::Thermalize
+entry points over
 GaugeLinks/ElectricFields runs on the GPU and matches the oracle."""
 import os
 import subprocess
@@ -101,3 +102,9 @@ def test_dropin_matches_oracle(nc, tmp_path):
     back = lat.copy()
     back.U[:], back.E[:] = U15, E15
     assert abs(float(lines[3]) - back.ncsdot()) < 1e-13
+    # InitialConditions::SetZero + ThermalDynamics::Thermalize(2, 1.6), beta 4, key MySEED = 7
+    th = O.Lattice(nc, N)
+    rows = th.thermalize(2, 1.6, 0.0375, 4.0, 7)
+    np.testing.assert_allclose(np.fromfile(tmp_path / "out_TU.bin"), th.U, atol=1e-11)
+    np.testing.assert_allclose(np.fromfile(tmp_path / "out_TE.bin"), th.E, atol=1e-11)
+    np.testing.assert_allclose(np.loadtxt(tmp_path / "ThermalEnergyID7.txt"), rows, rtol=1e-5, atol=1e-9)

@@ -366,3 +366,27 @@ def test_continue_cooling_matches_oracle(nc, N):
     assert abs(d - acc) <= 1e-10 * max(abs(acc), 1e-12)
     np.testing.assert_allclose(log, np.array(rows), rtol=1e-10, atol=1e-13)
     np.testing.assert_allclose(gpu.get_links(T.U_COOL), work.U, atol=1e-12)
+
+
+@pytest.mark.parametrize("nc,N", [(2, (6, 5, 4)), (3, (4, 5, 6))])
+def test_thermalize_matches_oracle(nc, N):
+    """ThermalDynamics::Thermalize: thermal E reassignment (Philox stream 3),
+    Gauss-law restoration, leapfrog and the ThermalEnergy rows every 20 steps,
+    from unit links; then the FP32 form against the FP64 one."""
+    lat = O.Lattice(nc, N)
+    gpu = T.Solver(nc, N, T.FP64)
+    gpu.unit_links()
+    gpu.zero_efield()
+    args = (2, 1.6, 0.0375, 4.0, 2024)   # 2 cycles of 43 steps, rows at steps 20 and 40
+    rows_o = lat.thermalize(*args)
+    rows_g = gpu.thermalize(*args)
+    assert rows_g.shape == rows_o.shape == (4, 7)
+    np.testing.assert_allclose(rows_g, rows_o, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-11)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-11)
+    g32 = T.Solver(nc, N, T.FP32)
+    g32.unit_links()
+    g32.zero_efield()
+    rows_32 = g32.thermalize(*args)
+    np.testing.assert_allclose(rows_32[:, 1:], rows_g[:, 1:], rtol=2e-5, atol=1e-7)
+    np.testing.assert_array_equal(rows_32[:, 0], rows_g[:, 0])

@@ -345,3 +345,34 @@ def test_blocking_adds_abelian_phases():
                 for mu in range(3):
                     t = th[f + (mu,)] + th[nb[mu] + (mu,)]
                     np.testing.assert_allclose(Ub[z, y, x, mu], [0, 0, np.sin(t / 2), np.cos(t / 2)], atol=1e-14)
+
+
+@pytest.mark.parametrize("nc,beta", [(2, 50.0), (3, 80.0)])
+def test_thermalization_reaches_equipartition(nc, beta):
+    """ThermalDynamics::Thermalize (DYNAMICS/ThermalDynamics.cpp:260-316) from
+    unit links at weak coupling: within a cycle T00 does not drift (it
+    fluctuates at O(omega dt), E being taken half a step from U);
+    the time-averaged magnetic and electric energies of the last cycle are
+    equal (virial theorem of the near-harmonic transverse modes), and the
+    electric energy grows from cycle to cycle towards n_E/beta, the value of
+    2 n_E transverse modes per site at temperature 1/beta."""
+    lat = O.Lattice(nc, (8, 8, 8))
+    cycles, max_time, dt = 4, 12.0, 0.0375
+    log = lat.thermalize(cycles, max_time, dt, beta, 123)
+    per = int(np.ceil(max_time / dt)) // 20
+    assert log.shape == (cycles * per, 7)
+    t = np.concatenate([np.arange(1, per + 1) * 20 * dt + i * max_time for i in range(cycles)])
+    np.testing.assert_allclose(log[:, 0], t, rtol=1e-15)
+    el, mg = [], []
+    for i in range(cycles):
+        blk = log[i * per:(i + 1) * per]
+        m = blk[:, 1].mean()
+        assert np.ptp(blk[:, 1]) < 0.05 * m
+        assert abs(blk[:per // 2, 1].mean() - blk[per // 2:, 1].mean()) < 5e-3 * m
+        np.testing.assert_allclose(blk[:, 1], blk[:, 5] + blk[:, 6], rtol=1e-13)
+        np.testing.assert_allclose(blk[:, 2] + blk[:, 3] + blk[:, 4], blk[:, 1], rtol=1e-12)
+        el.append(blk[:, 5].mean())
+        mg.append(blk[:, 6].mean())
+    assert abs(mg[-1] / el[-1] - 1.0) < 0.02
+    assert all(b > a for a, b in zip(el, el[1:]))
+    assert 0.8 < el[-1] / (O.ne(nc) / beta) < 1.0

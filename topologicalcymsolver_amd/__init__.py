@@ -31,7 +31,7 @@ EXPORTS = [
     "tcym_noise", "tcym_flow", "tcym_cool", "tcym_cooling_start", "tcym_cooling_update", "tcym_reunitarize",
     "tcym_edotb", "tcym_ncsdot", "tcym_edotb_map", "tcym_bulk", "tcym_hard_scales", "tcym_gauss", "tcym_unitarity",
     "tcym_gauss_restore", "tcym_geodesic", "tcym_delta_ncs_cooling", "tcym_vacuum_ncs",
-    "tcym_block", "tcym_level", "tcym_cool_blocked", "tcym_continue_cooling",
+    "tcym_block", "tcym_level", "tcym_cool_blocked", "tcym_continue_cooling", "tcym_thermalize",
     "tcym_timing", "tcym_kernel_time", "tcym_timing_reset", "tcym_sync", "tcym_device_bytes",
     "tcym_get_params",
 ]
@@ -134,6 +134,7 @@ def load_library(path=LIB_PATH):
         "tcym_batch_tune": [V, I, D, I, ctypes.POINTER(I), _dp], "tcym_batch_timing": [V, I],
         "tcym_batch_kernel_time": [V, ctypes.c_char_p, _dp, LL], "tcym_batch_timing_reset": [V],
         "tcym_get_params": [V, P],
+        "tcym_thermalize": [V, I, I, I, D, D, D, U64, I, D, _dp, I, ctypes.POINTER(I)],
         "tcym_write_text_links": [I, ctypes.POINTER(I), _dp, ctypes.c_char_p, I, I],
         "tcym_write_text_efield": [I, ctypes.POINTER(I), _dp, ctypes.c_char_p, I, I],
         "tcym_read_text_links": [I, ctypes.POINTER(I), _dp, ctypes.c_char_p, I],
@@ -385,6 +386,17 @@ class Solver:
                                               block_frequency, max_blocking, ctypes.byref(nb), ctypes.byref(d),
                                               log.ctypes.data_as(_dp)))
         return d.value, nb.value, log
+
+    def thermalize(self, cycles, max_time, dt=0.0375, beta=2.0, seed=0, maxiter=10000, tol=1e-14, u=U_HOT, e=E_HOT):
+        """ThermalDynamics::Thermalize (DYNAMICS/ThermalDynamics.cpp:260-316); returns the
+        ThermalEnergy rows {time, T00, TXX, TYY, TZZ, ELECTRIC, MAGNETIC} as an (n, 7) array."""
+        cap = cycles * (int(max_time / dt) // 20 + 2)
+        log = np.zeros(7 * max(cap, 1))
+        n = ctypes.c_int(0)
+        self._ck(self.L.tcym_thermalize(self.h, u, e, cycles, max_time, dt, beta, seed, maxiter, tol,
+                                        log.ctypes.data_as(_dp), cap, ctypes.byref(n)))
+        assert n.value <= cap
+        return log[:7 * n.value].reshape(n.value, 7)
 
     def reunitarize(self, u=U_HOT):
         self._ck(self.L.tcym_reunitarize(self.h, u))

@@ -709,6 +709,61 @@ int tcym_continue_cooling(tcym_ctx* c, int u_src, int u_work, int e_work, int st
     return perform_cooling(c, u_work, e_work, step0, nsteps, dtau, block_frequency, max_blocking, nblocked, 1, ctau0, dncs, log);
 }
 
+// ---------------------------------------------------------------- thermal initial condition
+int tcym_thermalize(tcym_ctx* c, int u, int e, int cycles, double max_time, double dt, double beta, uint64_t seed,
+                    int maxiter, double tol, double* log, int log_rows, int* nrows) {
+    if (!c) return TCYM_E_ARG;
+    if (cycles < 0 || !(dt > 0) || !(beta > 0) || !(max_time >= 0) || log_rows < 0 || (log_rows > 0 && !log))
+        return c->fail(TCYM_E_ARG, "tcym_thermalize: bad cycle count, step, beta, time or log");
+    int r;
+    Geom q;
+    if ((r = c->useU(u)) || (r = c->useE(e))) return r;
+    c->elev[e] = 0;
+    if ((r = c->at(q, u, e))) return r;
+    if (c->ulev[u] != 0) return c->fail(TCYM_E_ARG, "tcym_thermalize: the links must be at full resolution");
+    // Bulk::SetConstants, Minkowski (OBSERVABLES/BulkObservables.cpp:17-25)
+    double cb[3];
+    for (int mu = 0; mu < 3; mu++) {
+        const double x = q.a[mu]*q.aScale*q.aScale/(q.a[0]*q.a[1]*q.a[2]);
+        cb[mu] = x*x;
+    }
+    const double sigma = 1.0/sqrt(beta);
+    int rows = 0;
+    for (int i = 0; i < cycles; i++) {
+        // ReassignElectricFields: E^a ~ N(0, 1/beta), Philox stream 3 with the
+        // cycle as counter step, then GaussLawRestoration::Restore (DYNAMICS/ThermalDynamics.cpp:205-226)
+        c->be->random_efield(c->L(), q, seed, 3u, (uint32_t)i, sigma, c->E[e]);
+        if ((r = tcym_gauss_restore(c, u, e, maxiter, tol, nullptr, nullptr))) return r;
+        // evolve while Time() = tSteps dTau < MaxTime; Bulk every 20 steps (:274-292), Reset (:85-91)
+        long t = 0;
+        while ((double)t*dt < max_time) {
+            int n = 0;
+            do { t++; n++; } while (t % 20 != 0 && (double)t*dt < max_time);
+            if ((r = tcym_leapfrog(c, u, e, n, dt))) return r;
+            if (t % 20 == 0) {
+                double b[6];
+                if ((r = tcym_bulk(c, u, e, b))) return r;
+                if (rows < log_rows) {
+                    double* w = log + 7*(size_t)rows;
+                    const double E0 = cb[0]*b[0], E1 = cb[1]*b[1], E2 = cb[2]*b[2];
+                    const double B0 = cb[0]*b[3], B1 = cb[1]*b[4], B2 = cb[2]*b[5];
+                    // Time() + i MaxTime, T00, TXX, TYY, TZZ, ELECTRIC, MAGNETIC (BulkObservables.cpp:28-52)
+                    w[0] = (double)t*dt + i*max_time;
+                    w[1] = 0.5*(E0 + E1 + E2 + B0 + B1 + B2);
+                    w[2] = 0.5*((E1 + E2 + B1 + B2) - (E0 + B0));
+                    w[3] = 0.5*((E0 + E2 + B0 + B2) - (E1 + B1));
+                    w[4] = 0.5*((E0 + E1 + B0 + B1) - (E2 + B2));
+                    w[5] = 0.5*(E0 + E1 + E2);
+                    w[6] = 0.5*(B0 + B1 + B2);
+                }
+                rows++;
+            }
+        }
+    }
+    if (nrows) *nrows = rows;
+    return TCYM_OK;
+}
+
 // ---------------------------------------------------------------- timing
 int tcym_timing(tcym_ctx* c, int enable) {
     if (!c) return TCYM_E_ARG;
