@@ -5,12 +5,26 @@
 //   SU(3): 9 complex, column-major (i,j) -> i+3j, stored re,im (SUNc/SU3/GroupOperations.cpp:4-14)
 //   algebra: t^a = sigma_a/2 or lambda_a/2 (Gell-Mann order)
 // Every routine is templated on the arithmetic type T (double for the FP64
-// form, float for the FP32 form); reunitarisation always runs in double.
+// form, float for the FP32 form).  Reunitarisation runs in Reunit<T>: double,
+// or float for the FP32 form when TCYM_REUNIT_F32 is 1.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
+
+#ifndef TCYM_REUNIT_F32
+#define TCYM_REUNIT_F32 0
+#endif
 
 namespace tcym {
+
+template <typename T> struct ReunitOf { using type = double; };
+template <> struct ReunitOf<float> { using type = typename std::conditional<TCYM_REUNIT_F32 != 0, float, double>::type; };
+template <typename T> using Reunit = typename ReunitOf<T>::type;
+// 1/sqrt(x): correctly rounded division and square root in double; the
+// hardware reciprocal square root (about 1 ulp) in float
+__device__ __forceinline__ double rsqrt_of(double x) { return 1.0/__builtin_sqrt(x); }
+__device__ __forceinline__ float  rsqrt_of(float x)  { return __builtin_amdgcn_rsqf(x); }
 
 // 1/k!, k = 0..16
 __device__ constexpr double kInvFact[17] = {
@@ -108,10 +122,11 @@ template <typename T> struct SU2 {
         Mat C = mul_nd(U, U);
         return dsqrt(C.v[0]*C.v[0] + C.v[1]*C.v[1] + C.v[2]*C.v[2] + (C.v[3] - T(1))*(C.v[3] - T(1)));
     }
-    // FP64 normalisation of the quaternion
+    // normalisation of the quaternion, in Reunit<T> arithmetic
     __device__ static __forceinline__ void reunitarize(Mat& U) {
-        double a = U.v[0], b = U.v[1], c = U.v[2], d = U.v[3];
-        double n = 1.0/__builtin_sqrt(a*a + b*b + c*c + d*d);
+        using R = Reunit<T>;
+        R a = U.v[0], b = U.v[1], c = U.v[2], d = U.v[3];
+        R n = rsqrt_of(a*a + b*b + c*c + d*d);
         U.v[0] = T(a*n); U.v[1] = T(b*n); U.v[2] = T(c*n); U.v[3] = T(d*n);
     }
     __device__ static __forceinline__ T fstruct(int a, int b, int c) { return T(((a-b)*(a-c)*(c-b))/2); }
@@ -337,18 +352,19 @@ template <typename T> struct SU3 {
         }
         return dsqrt(n);
     }
-    // FP64 Gram-Schmidt on rows 0,1; row 2 = conj(row0 x row1)
+    // Gram-Schmidt on rows 0,1; row 2 = conj(row0 x row1), in Reunit<T> arithmetic
     __device__ static __forceinline__ void reunitarize(Mat& U) {
-        double r0r[3], r0i[3], r1r[3], r1i[3];
+        using R = Reunit<T>;
+        R r0r[3], r0i[3], r1r[3], r1i[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) { r0r[j] = U.r[3*j]; r0i[j] = U.i[3*j]; r1r[j] = U.r[1 + 3*j]; r1i[j] = U.i[1 + 3*j]; }
-        double n = 0;
+        R n = 0;
 #pragma unroll
         for (int j = 0; j < 3; j++) n += r0r[j]*r0r[j] + r0i[j]*r0i[j];
-        n = 1.0/__builtin_sqrt(n);
+        n = rsqrt_of(n);
 #pragma unroll
         for (int j = 0; j < 3; j++) { r0r[j] *= n; r0i[j] *= n; }
-        double pr = 0, pi = 0;   // <r0, r1>
+        R pr = 0, pi = 0;   // <r0, r1>
 #pragma unroll
         for (int j = 0; j < 3; j++) { pr += r0r[j]*r1r[j] + r0i[j]*r1i[j]; pi += r0r[j]*r1i[j] - r0i[j]*r1r[j]; }
         n = 0;
@@ -358,14 +374,14 @@ template <typename T> struct SU3 {
             r1i[j] -= pr*r0i[j] + pi*r0r[j];
             n += r1r[j]*r1r[j] + r1i[j]*r1i[j];
         }
-        n = 1.0/__builtin_sqrt(n);
+        n = rsqrt_of(n);
 #pragma unroll
         for (int j = 0; j < 3; j++) { r1r[j] *= n; r1i[j] *= n; }
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const int a = (j + 1) % 3, b = (j + 2) % 3;
-            double cr = r0r[a]*r1r[b] - r0i[a]*r1i[b] - (r0r[b]*r1r[a] - r0i[b]*r1i[a]);
-            double ci = r0r[a]*r1i[b] + r0i[a]*r1r[b] - (r0r[b]*r1i[a] + r0i[b]*r1r[a]);
+            R cr = r0r[a]*r1r[b] - r0i[a]*r1i[b] - (r0r[b]*r1r[a] - r0i[b]*r1i[a]);
+            R ci = r0r[a]*r1i[b] + r0i[a]*r1r[b] - (r0r[b]*r1i[a] + r0i[b]*r1r[a]);
             U.r[3*j] = T(r0r[j]); U.i[3*j] = T(r0i[j]);
             U.r[1 + 3*j] = T(r1r[j]); U.i[1 + 3*j] = T(r1i[j]);
             U.r[2 + 3*j] = T(cr); U.i[2 + 3*j] = T(-ci);
