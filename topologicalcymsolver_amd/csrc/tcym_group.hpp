@@ -22,9 +22,12 @@ template <typename T> struct ReunitOf { using type = double; };
 template <> struct ReunitOf<float> { using type = typename std::conditional<TCYM_REUNIT_F32 != 0, float, double>::type; };
 template <typename T> using Reunit = typename ReunitOf<T>::type;
 // 1/sqrt(x): correctly rounded division and square root in double; the
-// hardware reciprocal square root (about 1 ulp) in float
+// hardware reciprocal square root refined by one Newton step in float
 __device__ __forceinline__ double rsqrt_of(double x) { return 1.0/__builtin_sqrt(x); }
-__device__ __forceinline__ float  rsqrt_of(float x)  { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float  rsqrt_of(float x) {
+    const float r = __builtin_amdgcn_rsqf(x);
+    return r*(1.5f - 0.5f*x*r*r);   // one Newton step on the hardware estimate
+}
 
 // 1/k!, k = 0..16
 __device__ constexpr double kInvFact[17] = {
@@ -257,6 +260,33 @@ template <typename T> struct SU3 {
         A.r[7] = A.r[5]; A.i[7] = -A.i[5];      // (1,2)
         return A;
     }
+    // Q^2 of a hermitian Q with a real diagonal: the diagonal and the lower
+    // triangle, the upper triangle by conjugation (half a general product)
+    __device__ static __forceinline__ Mat herm_sq(const Mat& Q) {
+        Mat P;
+        P.r[0] = Q.r[0]*Q.r[0] + (Q.r[3]*Q.r[3] + Q.i[3]*Q.i[3]) + (Q.r[6]*Q.r[6] + Q.i[6]*Q.i[6]);
+        P.r[4] = (Q.r[1]*Q.r[1] + Q.i[1]*Q.i[1]) + Q.r[4]*Q.r[4] + (Q.r[7]*Q.r[7] + Q.i[7]*Q.i[7]);
+        P.r[8] = (Q.r[2]*Q.r[2] + Q.i[2]*Q.i[2]) + (Q.r[5]*Q.r[5] + Q.i[5]*Q.i[5]) + Q.r[8]*Q.r[8];
+        P.i[0] = T(0); P.i[4] = T(0); P.i[8] = T(0);
+        // (r, c) = sum_k Q(r, k) Q(k, c), element (r, c) at r + 3 c
+        auto el = [&](int r, int c, T& re, T& im) {
+            re = T(0); im = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const T ar = Q.r[r + 3*k], ai = (r == k) ? T(0) : Q.i[r + 3*k];
+                const T br = Q.r[k + 3*c], bi = (k == c) ? T(0) : Q.i[k + 3*c];
+                re += ar*br - ai*bi;
+                im += ar*bi + ai*br;
+            }
+        };
+        el(1, 0, P.r[1], P.i[1]);
+        el(2, 0, P.r[2], P.i[2]);
+        el(2, 1, P.r[5], P.i[5]);
+        P.r[3] = P.r[1]; P.i[3] = -P.i[1];
+        P.r[6] = P.r[2]; P.i[6] = -P.i[2];
+        P.r[7] = P.r[5]; P.i[7] = -P.i[5];
+        return P;
+    }
     // o^a = tr(lambda_a X) for hermitian X
     __device__ static __forceinline__ void herm_alg(const Mat& X, T* o) {
         const T s3 = T(0.57735026918962576450914878050196);
@@ -328,10 +358,9 @@ template <typename T> struct SU3 {
             T n2r = T(2)*m02r + m11r + m22r*cs1, n2i = T(2)*m02i + m11i + m22i*cs1;
             f0r = n0r; f0i = n0i; f1r = n1r; f1i = n1i; f2r = n2r; f2i = n2i;
         }
-        // U = f0 + f1 Qs + f2 Qs^2, Qs = sc Q
-#pragma unroll
-        for (int k = 0; k < 9; k++) { Q.r[k] *= sc; Q.i[k] *= sc; }
-        Mat Q2 = mul(Q, Q);
+        // U = f0 + f1 Qs + f2 Qs^2, Qs = sc Q (hermitian: its square from one triangle)
+        Q = alg_matrix(c*sc, al);
+        const Mat Q2 = herm_sq(Q);
         Mat U;
 #pragma unroll
         for (int k = 0; k < 9; k++) {
