@@ -180,6 +180,16 @@ int tcym_continue_cooling(tcym_ctx *ctx, int u_src, int u_work, int e_work, int 
    from their current values (the reference starts from InitialConditions::SetZero). */
 int tcym_thermalize(tcym_ctx *ctx, int u, int e, int cycles, double max_time, double dt, double beta, uint64_t seed,
                     int maxiter, double tol, double *log, int log_rows, int *nrows);
+/* InitialConditions::SetQuasiParticles without its final checks and Gauss-law
+   restoration (INITIALCONDITIONS/SetQuasiParticles.cpp:24-174, Minkowski metric):
+   momentum modes A_mu(p), E_mu(p) of occupancy f(p) = n0/sqrt(p^2/Qs^2 + 0.1) for
+   |p| < Qs along the free-field polarisations (MISC/MinkowskiWaveVectors.cpp),
+   an inverse FFT (rocFFT in place of FFTW), then U_mu = exp(-i A_mu^a t^a) and E_mu
+   from the real parts into link slot u and field slot e.  The complex normals of
+   the modes come from Philox stream 4 (counter colour + NE p, key seed) in place
+   of RandomNumberGenerator::ComplexGauss.  Follow with tcym_gauss_restore, as
+   the reference does. */
+int tcym_quasi_particles(tcym_ctx *ctx, int u, int e, double Qs, double n0, uint64_t seed);
 /* FP64 reunitarisation of every link (no reference counterpart: the FP32 form applies it after each step) */
 int tcym_reunitarize(tcym_ctx *ctx, int u);
 

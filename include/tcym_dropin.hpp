@@ -15,6 +15,7 @@
 //   GaussLawRestoration::Restore                               GAUSSLAWFIXING/GaussRestore.cpp:131
 //   IO::SaveConfiguration / LoadConfiguration                  IO/SaveConfiguration.cpp:3, IO/LoadConfiguration.cpp:3
 //   InitialConditions::SetZero, ThermalDynamics::Thermalize    INITIALCONDITIONS/SetZero.cpp:3, DYNAMICS/ThermalDynamics.cpp:260
+//   InitialConditions::SetQuasiParticles                       INITIALCONDITIONS/SetQuasiParticles.cpp:174
 // includes this header instead of those sources and links libtcym.so.
 //
 // Group and precision are run-time values (tcym_dropin::Configure) instead of
@@ -657,6 +658,30 @@ inline void LoadConfiguration(GaugeLinks* U, ElectricFields* E) {
 
 // ---------------------------------------------------------------- initial conditions
 namespace InitialConditions {
+// the quasi-particle parameters, globals of the drivers (SphaleronSimulationSU3.cpp:76-77)
+inline DOUBLE& Qs() { static DOUBLE v = 1.0; return v; }
+inline DOUBLE& n0() { static DOUBLE v = 1.0; return v; }
+// InitialConditions::SetQuasiParticles (INITIALCONDITIONS/SetQuasiParticles.cpp:24-174):
+// the modes, inverse FFT and exponentials on the device, then the
+// reference's observable printout and Gauss-law restoration; the complex
+// normals use RandomNumberGenerator::MySEED as their Philox key
+inline void SetQuasiParticles(GaugeLinks* U, ElectricFields* E) {
+    auto& d = tcym_dropin::dev(U);
+    const int u = d.links(U, TCYM_U_HOT), e = d.field(E, TCYM_E_HOT);
+    std::cerr << "#SETTING QUASI-PARTICLE INITIAL CONDITIONS" << std::endl;
+    d.check(tcym_quasi_particles(d.ctx, u, e, Qs(), n0(), (uint64_t)RandomNumberGenerator::MySEED()));
+    d.wrote_links(u);
+    d.wrote_field(e);
+    Observables::Bulk::Update(U, E);
+    std::cerr << "#INITIAL ENERGY DENSITIES AND PRESSURES -- T00 TXX TYY TZZ" << std::endl;
+    std::cerr << Observables::Bulk::T00() << " " << Observables::Bulk::TXX() << " " << Observables::Bulk::TYY() << " "
+              << Observables::Bulk::TZZ() << std::endl;
+    GaussLawRestoration::Restore(U, E);
+    Observables::Bulk::Update(U, E);
+    std::cerr << "#INITIAL ENERGY DENSITIES AND PRESSURES -- T00 TXX TYY TZZ" << std::endl;
+    std::cerr << Observables::Bulk::T00() << " " << Observables::Bulk::TXX() << " " << Observables::Bulk::TYY() << " "
+              << Observables::Bulk::TZZ() << std::endl;
+}
 // InitialConditions::SetZero (INITIALCONDITIONS/SetZero.cpp:3-24): unit links, zero E
 inline void SetZero(GaugeLinks* U, ElectricFields* E) {
     auto& d = tcym_dropin::dev(U);

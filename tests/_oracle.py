@@ -328,3 +328,61 @@ def from_matrix(nc, M):
 
 def alg_matrix(nc, alpha):
     return sum(alpha[a] * t for a, t in enumerate(generators(nc)))
+
+
+def quasi_particles(lat, Qs, n0, seed):
+    """InitialConditions::SetQuasiParticles before its Gauss-law restoration
+    (INITIALCONDITIONS/SetQuasiParticles.cpp:24-174, Minkowski metric), in
+    numpy: lattice momenta (LATTICE/Momenta.cpp:4-8), the polarisation vectors
+    of MISC/MinkowskiWaveVectors.cpp:6-58, the occupancy n0/sqrt(p^2/Qs^2 + 0.1)
+    below Qs (SetQuasiParticles.cpp:27-39), complex normals from Philox stream 4
+    (counter colour + NE p; the reference draws RandomNumberGenerator::ComplexGauss),
+    the unnormalised inverse FFT (FFTW_BACKWARD; numpy's ifftn times V), and
+    U = exp(-i A^a t^a), E from twice the real parts (SetQuasiParticles.cpp:119-165).
+    Sets lat.U and lat.E and returns (Xi1, Xi2, cDp) for checks."""
+    N = lat.N
+    V, nE = lat.V, ne(lat.nc)
+    a = np.array(list(lat.p.a))
+    ac = float(np.prod(a))
+    kz, ky, kx = np.meshgrid(np.arange(N[2]), np.arange(N[1]), np.arange(N[0]), indexing="ij")
+    k = (kx, ky, kz)
+    cDp = np.array([-1j * (1.0 - np.exp(-1j * 2.0 * np.pi * k[m] / N[m])) / a[m] for m in range(3)])
+    psq = (np.abs(cDp) ** 2).sum(axis=0)
+    om = np.sqrt(psq)
+    bv = np.array([1.0 / np.sqrt(3.0), 1.0 / np.sqrt(2.0), 1.0 / (np.sqrt(2.0) * np.sqrt(3.0))])
+    nz = psq > 0
+    safe = np.where(nz, psq, 1.0)
+    bdp = sum(bv[m] * cDp[m] for m in range(3))
+    nrm = np.sqrt(np.abs((bv ** 2).sum() - (bdp * np.conj(bdp)).real / safe))
+    den = nrm * np.sqrt(2.0 * np.where(nz, om, 1.0))
+    Xi1 = np.array([np.where(nz, (bv[m] - bdp * np.conj(cDp[m]) / safe) / den, 0.0) for m in range(3)])
+    Xi2 = np.array([np.where(nz, (cDp[(m + 1) % 3] * np.conj(Xi1[(m + 2) % 3])
+                                  - cDp[(m + 2) % 3] * np.conj(Xi1[(m + 1) % 3])) / np.where(nz, om, 1.0), 0.0)
+                    for m in range(3)])
+    pv2 = lat.p.aScale ** 2 * psq
+    f = np.where(pv2 < Qs * Qs, n0 / np.sqrt(pv2 / (Qs * Qs) + 0.1), 0.0)
+    amp = np.sqrt(f) * np.sqrt(V * ac)
+    L = lib()
+    z = np.zeros(4)
+    c1 = np.zeros((V, nE), complex)
+    c2 = np.zeros((V, nE), complex)
+    for p in range(V):
+        for c in range(nE):
+            L.or_link_normals(seed, 4, 0, c + nE * p, 4, _ptr(z))
+            c1[p, c] = complex(z[0], z[1]) / np.sqrt(2.0)
+            c2[p, c] = complex(z[2], z[3]) / np.sqrt(2.0)
+    shp = (N[2], N[1], N[0])
+    lat.E[:] = 0.0
+    U = lat.U.reshape(V, 3, -1)
+    E = lat.E.reshape(V, 3, nE)
+    for m in range(3):
+        x1, x2 = Xi1[m].reshape(-1), Xi2[m].reshape(-1)
+        Am = amp.reshape(-1)[:, None] * (c1 * x1[:, None] + c2 * x2[:, None])
+        Em = 1j * om.reshape(-1)[:, None] * Am
+        XA = np.fft.ifftn(Am.T.reshape((nE,) + shp), axes=(1, 2, 3)).reshape(nE, V).T * V
+        XE = np.fft.ifftn(Em.T.reshape((nE,) + shp), axes=(1, 2, 3)).reshape(nE, V).T * V
+        Al = 2.0 * a[m] * XA.real / (V * ac)
+        E[:, m, :] = 2.0 * (ac / a[m]) * XE.real / (V * ac)
+        for site in range(V):
+            L.or_expi(lat.nc, -1.0, _ptr(np.ascontiguousarray(Al[site])), _ptr(U[site, m]))
+    return Xi1, Xi2, cDp

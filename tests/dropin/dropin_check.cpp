@@ -9,7 +9,8 @@
 // (printing DeltaNCsCalibration, blocking number and flow clock; the log goes
 // to log_dir), saves the hot fields as the reference's text files in log_dir,
 // loads them into new containers and prints their NCsDot; finally thermalizes
-// a zero-field lattice into *_TU / *_TE (ThermalEnergy rows in log_dir).
+// a zero-field lattice into *_TU / *_TE (ThermalEnergy rows in log_dir) and
+// sets quasi-particle fields into *_QU / *_QE.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -96,6 +97,11 @@ int main(int argc, char** argv) {
         ThermalDynamics::Thermalize(2, 1.6, &UT, &ET);
         wr(std::string(argv[5]) + "_TU.bin", UT.U, nu);
         wr(std::string(argv[5]) + "_TE.bin", ET.E, ne);
+        // quasi-particles (Qs 1.5, n0 1, key MySEED = 7), Gauss law restored
+        InitialConditions::Qs() = 1.5;
+        InitialConditions::SetQuasiParticles(&UT, &ET);
+        wr(std::string(argv[5]) + "_QU.bin", UT.U, nu);
+        wr(std::string(argv[5]) + "_QE.bin", ET.E, ne);
     } catch (const tcym_dropin::Error& e) {
         fprintf(stderr, "%s (code %d)\n", e.what(), e.code);
         return 1;

@@ -1,8 +1,8 @@
 """The C++ drop-in (include/tcym_dropin.hpp): a program written against the
 reference's Dynamics::Update / NCsDot / Bulk / HardScales / GaussLaw /
 Unitarity / Cooling::CoolNSave / CoolingMethod::Calibrate /
-IO::SaveConfiguration / IO::LoadConfiguration / ThermalDynamics::Thermalize
-entry points over
+IO::SaveConfiguration / IO::LoadConfiguration / ThermalDynamics::Thermalize /
+InitialConditions::SetQuasiParticles entry points over
 GaugeLinks/ElectricFields runs on the GPU and matches the oracle."""
 import os
 import subprocess
@@ -108,3 +108,9 @@ def test_dropin_matches_oracle(nc, tmp_path):
     np.testing.assert_allclose(np.fromfile(tmp_path / "out_TU.bin"), th.U, atol=1e-11)
     np.testing.assert_allclose(np.fromfile(tmp_path / "out_TE.bin"), th.E, atol=1e-11)
     np.testing.assert_allclose(np.loadtxt(tmp_path / "ThermalEnergyID7.txt"), rows, rtol=1e-5, atol=1e-9)
+    # InitialConditions::SetQuasiParticles (Qs 1.5, n0 1, key 7) with its Gauss-law restoration
+    qp = O.Lattice(nc, N)
+    O.quasi_particles(qp, 1.5, 1.0, 7)
+    qp.gauss_restore()
+    np.testing.assert_allclose(np.fromfile(tmp_path / "out_QU.bin"), qp.U, atol=1e-12)
+    np.testing.assert_allclose(np.fromfile(tmp_path / "out_QE.bin"), qp.E, atol=1e-12)

@@ -390,3 +390,25 @@ def test_thermalize_matches_oracle(nc, N):
     rows_32 = g32.thermalize(*args)
     np.testing.assert_allclose(rows_32[:, 1:], rows_g[:, 1:], rtol=2e-5, atol=1e-7)
     np.testing.assert_array_equal(rows_32[:, 0], rows_g[:, 0])
+
+
+@pytest.mark.parametrize("nc,N,a", [(2, (6, 5, 4), (1.0, 1.0, 1.0)), (3, (4, 6, 5), (1.0, 1.25, 0.8))])
+def test_quasi_particles_match_oracle(nc, N, a):
+    """InitialConditions::SetQuasiParticles: momentum modes, rocFFT inverse
+    transform, exponentiated links and E against the numpy restatement; then
+    the FP32 form and the Gauss-law restoration the reference applies."""
+    lat = O.Lattice(nc, N, a=a, aScale=0.9)
+    O.quasi_particles(lat, 1.5, 1.0, 77)
+    gpu = T.Solver(nc, N, T.FP64, a=a, aScale=0.9)
+    gpu.quasi_particles(1.5, 1.0, 77)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-12)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-12)
+    assert gpu.unitarity_violation()[0] < 1e-13
+    g32 = T.Solver(nc, N, T.FP32, a=a, aScale=0.9)
+    g32.quasi_particles(1.5, 1.0, 77)
+    np.testing.assert_allclose(g32.get_links(), lat.U, atol=2e-7)
+    np.testing.assert_allclose(g32.get_efield(), lat.E, atol=1e-6 * np.abs(lat.E).max())
+    it, fm = gpu.gauss_law_restore()
+    it_o, fm_o = lat.gauss_restore()
+    assert it == it_o and fm <= 1e-14
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-12)

@@ -376,3 +376,34 @@ def test_thermalization_reaches_equipartition(nc, beta):
     assert abs(mg[-1] / el[-1] - 1.0) < 0.02
     assert all(b > a for a, b in zip(el, el[1:]))
     assert 0.8 < el[-1] / (O.ne(nc) / beta) < 1.0
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_quasi_particle_modes_are_transverse(nc):
+    """InitialConditions::SetQuasiParticles (INITIALCONDITIONS/SetQuasiParticles.cpp:24-174):
+    the polarisation vectors are transverse to the lattice momentum
+    (sum_k cDp_k Xi_k = 0, bilinear) and normalised to 1/(2 omega); hence, in
+    the abelian limit, E and A are divergence-free under the backward
+    difference at every site (the FFT is numpy's); with the non-abelian
+    parallel transport the Gauss-law violation grows like the amplitude
+    squared (occupancy scaled by 1/100: amplitude by 1/10, relative violation
+    by about 1/10)."""
+    N = (6, 5, 4)
+    lat = O.Lattice(nc, N)
+    Xi1, Xi2, cDp = O.quasi_particles(lat, 1.5, 1.0, 77)
+    om = np.sqrt((np.abs(cDp) ** 2).sum(axis=0))
+    nz = om > 0
+    for X in (Xi1, Xi2):
+        assert np.abs((cDp * X).sum(axis=0)).max() < 1e-14
+        np.testing.assert_allclose((np.abs(X) ** 2).sum(axis=0)[nz], 0.5 / om[nz], rtol=1e-12)
+    # abelian divergence of E and of the algebra of U (a = 1): sum_mu f_mu(x) - f_mu(x - mu)
+    E = lat.E.reshape(N[2], N[1], N[0], 3, -1)
+    div = sum(E[..., m, :] - np.roll(E[..., m, :], 1, axis=2 - m) for m in range(3))
+    assert np.abs(div).max() < 1e-12 * np.abs(E).max()
+    assert np.abs(E).max() > 0.01
+    # non-abelian Gauss law before restoration: relative violation ~ amplitude
+    g1 = lat.gauss()[1] / np.sqrt((lat.E ** 2).mean())
+    small = O.Lattice(nc, N)
+    O.quasi_particles(small, 1.5, 0.01, 77)
+    g2 = small.gauss()[1] / np.sqrt((small.E ** 2).mean())
+    assert 6.0 < g1 / g2 < 14.0, (g1, g2)

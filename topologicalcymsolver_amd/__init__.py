@@ -32,6 +32,7 @@ EXPORTS = [
     "tcym_edotb", "tcym_ncsdot", "tcym_edotb_map", "tcym_bulk", "tcym_hard_scales", "tcym_gauss", "tcym_unitarity",
     "tcym_gauss_restore", "tcym_geodesic", "tcym_delta_ncs_cooling", "tcym_vacuum_ncs",
     "tcym_block", "tcym_level", "tcym_cool_blocked", "tcym_continue_cooling", "tcym_thermalize",
+    "tcym_quasi_particles",
     "tcym_timing", "tcym_kernel_time", "tcym_timing_reset", "tcym_sync", "tcym_device_bytes",
     "tcym_get_params",
 ]
@@ -135,6 +136,7 @@ def load_library(path=LIB_PATH):
         "tcym_batch_kernel_time": [V, ctypes.c_char_p, _dp, LL], "tcym_batch_timing_reset": [V],
         "tcym_get_params": [V, P],
         "tcym_thermalize": [V, I, I, I, D, D, D, U64, I, D, _dp, I, ctypes.POINTER(I)],
+        "tcym_quasi_particles": [V, I, I, D, D, U64],
         "tcym_write_text_links": [I, ctypes.POINTER(I), _dp, ctypes.c_char_p, I, I],
         "tcym_write_text_efield": [I, ctypes.POINTER(I), _dp, ctypes.c_char_p, I, I],
         "tcym_read_text_links": [I, ctypes.POINTER(I), _dp, ctypes.c_char_p, I],
@@ -397,6 +399,11 @@ class Solver:
                                         log.ctypes.data_as(_dp), cap, ctypes.byref(n)))
         assert n.value <= cap
         return log[:7 * n.value].reshape(n.value, 7)
+
+    def quasi_particles(self, Qs=1.0, n0=1.0, seed=0, u=U_HOT, e=E_HOT):
+        """InitialConditions::SetQuasiParticles before its Gauss-law restoration
+        (INITIALCONDITIONS/SetQuasiParticles.cpp:24-174)."""
+        self._ck(self.L.tcym_quasi_particles(self.h, u, e, Qs, n0, seed))
 
     def reunitarize(self, u=U_HOT):
         self._ck(self.L.tcym_reunitarize(self.h, u))

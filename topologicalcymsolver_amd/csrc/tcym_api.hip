@@ -10,6 +10,10 @@
 #include <utility>
 #include <vector>
 
+#include <rocfft/rocfft.h>
+
+#include <mutex>
+
 #include "tcym.h"
 #include "tcym_backend.hpp"
 #include "tcym_reduce.hpp"
@@ -762,6 +766,53 @@ int tcym_thermalize(tcym_ctx* c, int u, int e, int cycles, double max_time, doub
     }
     if (nrows) *nrows = rows;
     return TCYM_OK;
+}
+
+// ---------------------------------------------------------------- quasi-particle initial condition
+int tcym_quasi_particles(tcym_ctx* c, int u, int e, double Qs, double n0, uint64_t seed) {
+    if (!c) return TCYM_E_ARG;
+    if (!(Qs > 0) || !(n0 >= 0)) return c->fail(TCYM_E_ARG, "tcym_quasi_particles: Qs must be positive and n0 not negative");
+    int r;
+    if ((r = c->useU(u)) || (r = c->useE(e))) return r;
+    c->ulev[u] = 0;
+    c->elev[e] = 0;
+    const Geom& g = c->g;
+    const size_t nmodes = (size_t)6*c->be->ne*(size_t)g.V;   // A_mu and E_mu of every colour
+    double* P = nullptr;
+    void* work = nullptr;
+    if ((r = c->hip(hipMalloc((void**)&P, sizeof(double)*2*nmodes), "hipMalloc(quasi-particle modes)"))) return r;
+    auto done = [&](int rr) {
+        hipStreamSynchronize(c->stream);
+        if (work) hipFree(work);
+        hipFree(P);
+        return rr;
+    };
+    c->be->qp_modes(c->L(), g, Qs, n0, seed, P);
+    // FourierSpace::*->ExecutePtoX (FFT/FFT3D.cpp, FFTW_BACKWARD: unnormalised,
+    // +i exponent): rocFFT's complex inverse transform, batched over 6 NE modes
+    static std::once_flag setup;
+    std::call_once(setup, [] { rocfft_setup(); });
+    rocfft_plan plan = nullptr;
+    rocfft_execution_info info = nullptr;
+    const size_t len[3] = {(size_t)g.n[0], (size_t)g.n[1], (size_t)g.n[2]};   // fastest (x) first
+    size_t wbytes = 0;
+    bool ok = rocfft_plan_create(&plan, rocfft_placement_inplace, rocfft_transform_type_complex_inverse,
+                                 rocfft_precision_double, 3, len, (size_t)6*c->be->ne, nullptr) == rocfft_status_success &&
+              rocfft_plan_get_work_buffer_size(plan, &wbytes) == rocfft_status_success &&
+              rocfft_execution_info_create(&info) == rocfft_status_success &&
+              rocfft_execution_info_set_stream(info, c->stream) == rocfft_status_success;
+    if (ok && wbytes > 0)
+        ok = hipMalloc(&work, wbytes) == hipSuccess &&
+             rocfft_execution_info_set_work_buffer(info, work, wbytes) == rocfft_status_success;
+    if (ok) {
+        void* in[1] = {P};
+        ok = rocfft_execute(plan, in, nullptr, info) == rocfft_status_success;
+    }
+    if (info) rocfft_execution_info_destroy(info);
+    if (plan) rocfft_plan_destroy(plan);
+    if (!ok) return done(c->fail(TCYM_E_HIP, "tcym_quasi_particles: rocFFT plan or execution failed"));
+    c->be->qp_fields(c->L(), g, P, c->U[u], c->E[e]);
+    return done(c->sync());
 }
 
 // ---------------------------------------------------------------- timing

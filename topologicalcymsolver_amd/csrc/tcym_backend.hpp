@@ -88,6 +88,11 @@ struct Backend {
     virtual void reunitarize(const Launcher&, const Geom&, void* U) = 0;
     // links of the fine lattice g -> the lattice of half the extent (Uc distinct from Uf)
     virtual void block_links(const Launcher&, const Geom& g, const void* Uf, void* Uc) = 0;
+    // quasi-particle initial condition: the 6 NE momentum-space modes (A_mu, E_mu
+    // of every colour, complex FP64, [field][colour][p]) and, after their
+    // inverse FFT, the links and fields from the real parts
+    virtual void qp_modes(const Launcher&, const Geom& g, double Qs, double n0, uint64_t seed, double* P) = 0;
+    virtual void qp_fields(const Launcher&, const Geom& g, const double* X, void* U, void* E) = 0;
 
     // batches of lattices (tcym_batch.hip): one persistent launch per pass
     // over every lattice of the batch, same per-link / per-site arithmetic as
