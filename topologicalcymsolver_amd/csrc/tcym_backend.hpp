@@ -54,7 +54,6 @@ struct BatchDev {
 
 struct Backend {
     int ns = 0, ne = 0, elsize = 0, nc = 0;
-    int update_tiles = 3;   // update-kernel blocks per block of sites: 3 (per link) or 1 (per site)
     virtual ~Backend() {}
     // layout conversions: host AoS FP64 (staged on the device) <-> device tiled layout
     virtual void pack_links(const Launcher&, const Geom&, const double* aos, void* soa) = 0;

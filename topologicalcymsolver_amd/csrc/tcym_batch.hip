@@ -160,7 +160,7 @@ struct tcym_batch : tcym::DeviceQueue {
             b.stile0 = (uint32_t)st.size();
             l0.push_back(b.ltile0); ln.push_back((uint32_t)(3*wr.blocks));
             s0.push_back(b.stile0); sn.push_back((uint32_t)wr.blocks);
-            ut.insert(ut.end(), (size_t)(be->update_tiles*wu.blocks), (uint32_t)e);
+            ut.insert(ut.end(), (size_t)(3*wu.blocks), (uint32_t)e);
             lt.insert(lt.end(), (size_t)(3*wr.blocks), (uint32_t)e);
             st.insert(st.end(), (size_t)wr.blocks, (uint32_t)e);
         }
@@ -170,7 +170,7 @@ struct tcym_batch : tcym::DeviceQueue {
         for (const Geom& q : geo) {
             long m = 0;
             for (int s : {-1, 0, 4, 5, 6}) m = std::max(m, update_shape(q, s).blocks);
-            ucap += (size_t)(be->update_tiles*m);
+            ucap += (size_t)(3*m);
         }
         int r;
         if ((r = alloc((void**)&utiles, sizeof(uint32_t)*ucap, "hipMalloc(tiles)"))) return r;

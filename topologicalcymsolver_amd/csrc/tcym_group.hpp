@@ -28,6 +28,25 @@ __device__ __forceinline__ float  rsqrt_of(float x) {
     const float r = __builtin_amdgcn_rsqf(x);
     return r*(1.5f - 0.5f*x*r*r);   // one Newton step on the hardware estimate
 }
+// 1/sqrt(x) of a squared row norm in the reunitarisation of a link stored in
+// T.  FP64 links: rsqrt_of.  FP32 links reunitarised in double: the hardware
+// double estimate refined by two Newton steps (9 instructions instead of the
+// 31 of a correctly rounded square root and division), a few double ulp from
+// 1/sqrt(x) for the x near 1 met here, so the rounded FP32 link is that of
+// the exact value but at a rounding boundary.
+template <typename T>
+__device__ __forceinline__ double rsqrt_reunit(double x) {
+    if constexpr (sizeof(T) == 8) {
+        return rsqrt_of(x);
+    } else {
+        const double h = 0.5*x;
+        double r = __builtin_amdgcn_rsq(x);
+        r = r*__builtin_fma(-h*r, r, 1.5);
+        return r*__builtin_fma(-h*r, r, 1.5);
+    }
+}
+template <typename T>
+__device__ __forceinline__ float rsqrt_reunit(float x) { return rsqrt_of(x); }
 
 // 1/k!, k = 0..16
 __device__ constexpr double kInvFact[17] = {
@@ -71,6 +90,9 @@ template <typename T> struct SU2 {
     __device__ static __forceinline__ Mat mul_nd(const Mat& A, const Mat& B) { return mul(A, dag(B)); }
     __device__ static __forceinline__ Mat mul_dn(const Mat& A, const Mat& B) { return mul(dag(A), B); }
     __device__ static __forceinline__ Mat mul_dd(const Mat& A, const Mat& B) { return mul(dag(A), dag(B)); }
+    // op(A) op(B) of two group elements (the SU(3) form saves work this way; here it is mul)
+    template <bool dA, bool dB>
+    __device__ static __forceinline__ Mat mulsu(const Mat& A, const Mat& B) { return mul(dA ? dag(A) : A, dB ? dag(B) : B); }
     __device__ static __forceinline__ void axpy(T s, const Mat& X, Mat& Y) {
 #pragma unroll
         for (int k = 0; k < 4; k++) Y.v[k] += s*X.v[k];
@@ -129,7 +151,7 @@ template <typename T> struct SU2 {
     __device__ static __forceinline__ void reunitarize(Mat& U) {
         using R = Reunit<T>;
         R a = U.v[0], b = U.v[1], c = U.v[2], d = U.v[3];
-        R n = rsqrt_of(a*a + b*b + c*c + d*d);
+        R n = rsqrt_reunit<T>(a*a + b*b + c*c + d*d);
         U.v[0] = T(a*n); U.v[1] = T(b*n); U.v[2] = T(c*n); U.v[3] = T(d*n);
     }
     __device__ static __forceinline__ T fstruct(int a, int b, int c) { return T(((a-b)*(a-c)*(c-b))/2); }
@@ -175,6 +197,46 @@ template <typename T> struct SU3 {
     __device__ static __forceinline__ Mat mul_nd(const Mat& A, const Mat& B) { return mulg<false, true>(A, B); }
     __device__ static __forceinline__ Mat mul_dn(const Mat& A, const Mat& B) { return mulg<true, false>(A, B); }
     __device__ static __forceinline__ Mat mul_dd(const Mat& A, const Mat& B) { return mulg<true, true>(A, B); }
+    // column 2 = conj(col0 x col1) from columns 0 and 1 (exact for special unitary m)
+    __device__ static __forceinline__ void complete(Mat& m) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int j = (i + 1) % 3, l = (i + 2) % 3;
+            // c0[j] c1[l] - c0[l] c1[j], conjugated
+            const T ar = m.r[j], ai = m.i[j], br = m.r[3 + l], bi = m.i[3 + l];
+            const T cr = m.r[l], ci = m.i[l], dr = m.r[3 + j], di = m.i[3 + j];
+            m.r[6 + i] = fmat(ar, br, fmat(-ai, bi, fmat(-cr, dr, ci*di)));
+            m.i[6 + i] = -fmat(ar, bi, fmat(ai, br, fmat(-cr, di, -(ci*dr))));
+        }
+    }
+    // op(A) op(B) of two special unitary matrices: columns 0 and 1 by the
+    // product (72 multiply-adds), column 2 rebuilt (24) instead of multiplied
+    // (36).  op(B)'s column 2 is not read, so for dB = false a B from ld2
+    // needs no rebuild at all, and for dB = true only two of its three
+    // rebuilt entries are read; the compiler drops the unread ones.
+    template <bool dA, bool dB>
+    __device__ static __forceinline__ Mat mulsu(const Mat& A, const Mat& B) {
+        Mat C;
+#pragma unroll
+        for (int row = 0; row < 3; row++) {
+#pragma unroll
+            for (int col = 0; col < 2; col++) {
+                T sr = T(0), si = T(0);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int ia = dA ? (k + 3*row) : (row + 3*k);
+                    const int ib = dB ? (col + 3*k) : (k + 3*col);
+                    const T ar = A.r[ia], ai = dA ? -A.i[ia] : A.i[ia];
+                    const T br = B.r[ib], bi = dB ? -B.i[ib] : B.i[ib];
+                    sr = fmat(ar, br, sr); sr = fmat(-ai, bi, sr);
+                    si = fmat(ar, bi, si); si = fmat(ai, br, si);
+                }
+                C.r[row + 3*col] = sr; C.i[row + 3*col] = si;
+            }
+        }
+        complete(C);
+        return C;
+    }
     __device__ static __forceinline__ Mat dag(const Mat& A) {
         Mat D;
 #pragma unroll
@@ -390,7 +452,7 @@ template <typename T> struct SU3 {
         R n = 0;
 #pragma unroll
         for (int j = 0; j < 3; j++) n += r0r[j]*r0r[j] + r0i[j]*r0i[j];
-        n = rsqrt_of(n);
+        n = rsqrt_reunit<T>(n);
 #pragma unroll
         for (int j = 0; j < 3; j++) { r0r[j] *= n; r0i[j] *= n; }
         R pr = 0, pi = 0;   // <r0, r1>
@@ -403,7 +465,7 @@ template <typename T> struct SU3 {
             r1i[j] -= pr*r0i[j] + pi*r0r[j];
             n += r1r[j]*r1r[j] + r1i[j]*r1i[j];
         }
-        n = rsqrt_of(n);
+        n = rsqrt_reunit<T>(n);
 #pragma unroll
         for (int j = 0; j < 3; j++) { r1r[j] *= n; r1i[j] *= n; }
 #pragma unroll
