@@ -107,9 +107,13 @@ def run_form(T, nc, N, precision, U, E, steps, warmup, dt, cool_every, cool_step
     t0 = time.perf_counter()
     g.cooling_method_start(cool_steps, dtau)
     n_flow += cool_steps
-    for k in range(1, steps + 1):
-        g.dynamics_update(1, dt)
-        n_lf += 1
+    k = 0
+    while k < steps:
+        # the leapfrog steps up to the next measurement in one call
+        n = min(cool_every - k % cool_every, steps - k)
+        g.dynamics_update(n, dt)
+        n_lf += n
+        k += n
         if k % cool_every == 0 or k == steps:
             d = g.cooling_method_update(cool_steps, dtau, measure=True)
             n_flow += cool_steps

@@ -347,7 +347,7 @@ static int run_update(tcym_ctx* c, int mode, int u, int e, int xi, int nsteps, d
     const void* X = mode == 2 ? c->E[xi] : nullptr;
     for (int k = 0; k < nsteps; k++) {
         c->be->update(c->L(), q, mode, true, write_e, dt, damping, noise_sd, seed, step0 + (uint32_t)k,
-                      c->U[u], c->Uscr, E, X);
+                      c->U[u], c->Uscr, E, X, k == nsteps - 1);
         std::swap(c->U[u], c->Uscr);
         if (c->last != hipSuccess) return TCYM_E_HIP;
     }
@@ -363,7 +363,7 @@ int tcym_update_efield(tcym_ctx* c, int u, int e, double dt) {
     int r;
     Geom q;
     if ((r = c->useU(u)) || (r = c->useE(e)) || (r = c->at(q, u, e))) return r;
-    c->be->update(c->L(), q, 0, false, true, dt, 0.0, 0.0, 0, 0, c->U[u], nullptr, c->E[e], nullptr);
+    c->be->update(c->L(), q, 0, false, true, dt, 0.0, 0.0, 0, 0, c->U[u], nullptr, c->E[e], nullptr, true);
     return c->sync();
 }
 int tcym_update_links(tcym_ctx* c, int u, int e, double dt) {
@@ -401,7 +401,8 @@ int tcym_evolve_ncs(tcym_ctx* c, int mode, int u, int e, int nsteps, double dt, 
     if ((r = c->hip(hipMemcpyAsync(acc, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(acc)"))) return r;
     const double half_dt = 0.5*dt;
     for (int k = 0; k < nsteps; k++) {
-        c->be->update(c->L(), q, mode, true, true, dt, damping, noise, seed, step0 + (uint32_t)k, c->U[u], c->Uscr, c->E[e], nullptr);
+        c->be->update(c->L(), q, mode, true, true, dt, damping, noise, seed, step0 + (uint32_t)k, c->U[u], c->Uscr, c->E[e], nullptr,
+                      true);
         std::swap(c->U[u], c->Uscr);     // Uscr now holds U_n
         long nb = c->be->edotb(c->L(), q, 1, c->Uscr, c->E[e], c->part, nullptr);
         tcym::final_reduce(c->L(), c->part, nb, 1, &mx, eb);
@@ -434,7 +435,7 @@ int tcym_flow(tcym_ctx* c, int u, int e, int nsteps, double dtau, int calibrate,
     tcym_ctx::flow_cu(q, dtau, cU);
     for (int k = 0; k < nsteps; k++) {
         // GradientFlow::CalibrationUpdate (GRADIENTFLOW/GradientFlow.cpp:168-188)
-        c->be->update(c->L(), q, 3, false, true, dtau, 0.0, 0.0, 0, 0, c->U[u], nullptr, c->E[e], nullptr);
+        c->be->update(c->L(), q, 3, false, true, dtau, 0.0, 0.0, 0, 0, c->U[u], nullptr, c->E[e], nullptr, true);
         if ((r = c->ncsdot(u, e, &v))) return r;
         acc += 0.5*D*v;
         c->be->link_update(c->L(), q, cU, c->U[u], c->Uscr, c->E[e]);

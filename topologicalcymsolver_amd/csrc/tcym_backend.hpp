@@ -70,9 +70,13 @@ struct Backend {
     virtual void random_efield(const Launcher&, const Geom&, uint64_t seed, uint32_t stream, uint32_t step, double sigma, void* E) = 0;
     // updates; Uin -> Uout (distinct buffers), E in place
     // mode: 0 leapfrog, 1 Langevin (Philox noise), 2 Langevin (Xi field), 3 flow (E := force)
+    // complete = false stores only columns 0 and 1 of SU(3) links: for an
+    // output that no kernel but the next update step reads (the update
+    // kernels load links as columns 0, 1 and rebuild column 2); the last step
+    // of a sequence stores complete links
     virtual void update(const Launcher&, const Geom&, int mode, bool links, bool write_e, double dt,
                         double damping, double noise_sd, uint64_t seed, uint32_t step,
-                        const void* Uin, void* Uout, void* E, const void* Xi) = 0;
+                        const void* Uin, void* Uout, void* E, const void* Xi, bool complete) = 0;
     // U <- exp(i cU_mu E) U (Uin -> Uout)
     virtual void link_update(const Launcher&, const Geom&, const double* cU, const void* Uin, void* Uout, const void* E) = 0;
     // reductions: per-block partials (double) written to `part`; return number of blocks
@@ -98,7 +102,8 @@ struct Backend {
     // over every lattice of the batch, same per-link / per-site arithmetic as
     // the single-lattice kernels.  Partials go to part[tile*nq + q].
     virtual void update_batch(const Launcher&, const BatchDev&, int mode, bool write_e, double dt, double damping,
-                              double noise_sd, uint32_t step_off, const void* Uin, void* Uout, void* E, const void* Xi) = 0;
+                              double noise_sd, uint32_t step_off, const void* Uin, void* Uout, void* E, const void* Xi,
+                              bool complete) = 0;
     virtual void edotb_batch(const Launcher&, const BatchDev&, const void* U, const void* E, double* part) = 0;
     virtual void bulk_batch(const Launcher&, const BatchDev&, const void* U, const void* E, double* part) = 0;
     virtual void gauss_batch(const Launcher&, const BatchDev&, const void* U, const void* E, double* part) = 0;

@@ -205,9 +205,11 @@ struct tcym_batch : tcym::DeviceQueue {
         drop_graphs();
         return sync();
     }
-    // one update step of every lattice: U[u] -> Uscr, then swap
-    void step(int mode, int u, int e, bool write_e, double dt, double damping, double noise_sd, uint32_t step_off) {
-        be->update_batch(L(), dev(), mode, write_e, dt, damping, noise_sd, step_off, U[u], Uscr, e >= 0 ? E[e] : nullptr, nullptr);
+    // one update step of every lattice: U[u] -> Uscr, then swap; `complete`
+    // false for a step whose links only the next update step reads
+    void step(int mode, int u, int e, bool write_e, double dt, double damping, double noise_sd, uint32_t step_off, bool complete) {
+        be->update_batch(L(), dev(), mode, write_e, dt, damping, noise_sd, step_off, U[u], Uscr, e >= 0 ? E[e] : nullptr, nullptr,
+                         complete);
         std::swap(U[u], Uscr);
     }
     void advance(uint32_t steps) {
@@ -250,7 +252,7 @@ struct tcym_batch : tcym::DeviceQueue {
     int flow(int u, int nsteps, double dtau) {
         int r;
         if ((r = useU(u)) || (r = scratch())) return r;
-        for (int k = 0; k < nsteps; k++) step(3, u, -1, false, dtau, 0.0, 0.0, 0);
+        for (int k = 0; k < nsteps; k++) step(3, u, -1, false, dtau, 0.0, 0.0, 0, k == nsteps - 1);
         return status();
     }
 };
@@ -421,7 +423,7 @@ static int run_updates(tcym_batch* b, int mode, int nsteps, double dt, double da
     if ((r = b->useU(TCYM_U_HOT)) || (r = b->useE(TCYM_E_HOT)) || (r = b->scratch())) return r;
     if (nsteps < 0) return b->fail(TCYM_E_ARG, "nsteps < 0");
     for (int k = 0; k < nsteps; k++) {
-        b->step(mode, TCYM_U_HOT, TCYM_E_HOT, true, dt, damping, noise_sd, (uint32_t)k);
+        b->step(mode, TCYM_U_HOT, TCYM_E_HOT, true, dt, damping, noise_sd, (uint32_t)k, k == nsteps - 1);
         if (b->last != hipSuccess) return TCYM_E_HIP;
     }
     if (mode == 1) b->advance((uint32_t)nsteps);
@@ -482,7 +484,7 @@ int tcym_batch_evolve(tcym_batch* b, int mode, int nseg, int steps, double dt, d
             void* s0 = b->Uscr;
             if ((r = b->hip(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))) return r;
             b->capturing = true;
-            for (int k = 0; k < steps; k++) b->step(mode, TCYM_U_HOT, TCYM_E_HOT, true, dt, damping, noise, (uint32_t)k);
+            for (int k = 0; k < steps; k++) b->step(mode, TCYM_U_HOT, TCYM_E_HOT, true, dt, damping, noise, (uint32_t)k, k == steps - 1);
             b->be->edotb_batch(b->L(), b->dev(), b->U[TCYM_U_HOT], b->E[TCYM_E_HOT], b->part);
             b->final_reduce(true, 1, 0u);
             const long cap = b->traj_cap;
@@ -620,11 +622,11 @@ int tcym_batch_tune(tcym_batch* b, int mode, double dt, int reps, int choice[3],
                 const tcym::BatchDev d = b->dev();
                 // one warm-up launch, then reps timed launches
                 b->be->update_batch(b->L(), d, mode, mode != 3, dt, 0.1, mode == 1 ? 0.1 : 0.0, 0,
-                                    b->U[TCYM_U_HOT], b->Uscr, b->E[TCYM_E_AUX], nullptr);
+                                    b->U[TCYM_U_HOT], b->Uscr, b->E[TCYM_E_AUX], nullptr, false);
                 hipEventRecord(e0, b->stream);
                 for (int k = 0; k < reps; k++)
                     b->be->update_batch(b->L(), d, mode, mode != 3, dt, 0.1, mode == 1 ? 0.1 : 0.0, 0,
-                                        b->U[TCYM_U_HOT], b->Uscr, b->E[TCYM_E_AUX], nullptr);
+                                        b->U[TCYM_U_HOT], b->Uscr, b->E[TCYM_E_AUX], nullptr, false);
                 hipEventRecord(e1, b->stream);
                 if ((r = b->sync())) break;
                 float ms = 0.f;
