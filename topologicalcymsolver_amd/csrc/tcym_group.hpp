@@ -31,9 +31,9 @@ __device__ __forceinline__ float  rsqrt_of(float x) {
 // 1/sqrt(x) of a squared row norm in the reunitarisation of a link stored in
 // T.  FP64 links: rsqrt_of.  FP32 links reunitarised in double: the hardware
 // double estimate refined by two Newton steps (9 instructions instead of the
-// 31 of a correctly rounded square root and division), a few double ulp from
-// 1/sqrt(x) for the x near 1 met here, so the rounded FP32 link is that of
-// the exact value but at a rounding boundary.
+// 31 of a correctly rounded square root and division), within a few double
+// ulp of 1/sqrt(x); the FP32 link it rounds to differs from the one of the
+// exact value only where that lies within those ulp of a float boundary.
 template <typename T>
 __device__ __forceinline__ double rsqrt_reunit(double x) {
     if constexpr (sizeof(T) == 8) {
