@@ -378,11 +378,12 @@ template <typename T> struct SU3 {
     // scaling Q by 2^-s so that c1 <= 1/4; the result is squared s times in
     // the same basis.
     __device__ static __forceinline__ Mat expi(T c, const T* al) {
-        Mat Q = alg_matrix(c, al);
+        const Mat Q = alg_matrix(c, al);
+        // c1 = tr(Q^2)/2 = (c^2/4) sum_a al_a^2, since tr(t^a t^b) = delta_ab/2
         T c1 = T(0);
 #pragma unroll
-        for (int k = 0; k < 9; k++) c1 += Q.r[k]*Q.r[k] + Q.i[k]*Q.i[k];
-        c1 *= T(0.5);
+        for (int a = 0; a < 8; a++) c1 = fmat(al[a], al[a], c1);
+        c1 *= T(0.25)*c*c;
         if (c1 == T(0)) return unit();
         // det of hermitian Q (real)
         T m0 = Q.r[4]*Q.r[8] - (Q.r[7]*Q.r[7] + Q.i[7]*Q.i[7]);          // Q11 Q22 - |Q12|^2
@@ -420,8 +421,10 @@ template <typename T> struct SU3 {
             T n2r = T(2)*m02r + m11r + m22r*cs1, n2i = T(2)*m02i + m11i + m22i*cs1;
             f0r = n0r; f0i = n0i; f1r = n1r; f1i = n1i; f2r = n2r; f2i = n2i;
         }
-        // U = f0 + f1 Qs + f2 Qs^2, Qs = sc Q (hermitian: its square from one triangle)
-        Q = alg_matrix(c*sc, al);
+        // U = f0 + f1 Qs + f2 Qs^2 with Qs = sc Q, as f0 + (sc f1) Q + (sc^2 f2) Q^2
+        // (Q hermitian: its square from one triangle)
+        f1r *= sc; f1i *= sc;
+        f2r *= sc*sc; f2i *= sc*sc;
         const Mat Q2 = herm_sq(Q);
         Mat U;
 #pragma unroll
