@@ -333,10 +333,16 @@ int tcym_level(tcym_ctx* c, int u, int e, int* level, int N[3]) {
 }
 
 // ---------------------------------------------------------------- updates
+// nsteps update steps of slot u; with src >= 0 the first step reads slot src
+// instead (slot u takes its level and ends with the result; src is unchanged)
 static int run_update(tcym_ctx* c, int mode, int u, int e, int xi, int nsteps, double dt, double damping,
-                      double noise_sd, uint64_t seed, uint32_t step0, bool write_e) {
+                      double noise_sd, uint64_t seed, uint32_t step0, bool write_e, int src = -1) {
     int r;
     if ((r = c->useU(u)) || (r = c->scratch())) return r;
+    if (src >= 0) {
+        if ((r = c->useU(src))) return r;
+        c->ulev[u] = c->ulev[src];
+    }
     if (e >= 0 && (r = c->useE(e))) return r;
     if (mode != 3 && e < 0) return c->fail(TCYM_E_ARG, "E slot required");
     if (mode == 2 && (r = c->useE(xi))) return r;
@@ -347,7 +353,7 @@ static int run_update(tcym_ctx* c, int mode, int u, int e, int xi, int nsteps, d
     const void* X = mode == 2 ? c->E[xi] : nullptr;
     for (int k = 0; k < nsteps; k++) {
         c->be->update(c->L(), q, mode, true, write_e, dt, damping, noise_sd, seed, step0 + (uint32_t)k,
-                      c->U[u], c->Uscr, E, X, k == nsteps - 1);
+                      k == 0 && src >= 0 ? c->U[src] : c->U[u], c->Uscr, E, X, k == nsteps - 1);
         std::swap(c->U[u], c->Uscr);
         if (c->last != hipSuccess) return TCYM_E_HIP;
     }
@@ -449,9 +455,12 @@ int tcym_flow(tcym_ctx* c, int u, int e, int nsteps, double dtau, int calibrate,
 int tcym_cool(tcym_ctx* c, int u_hot, int u_cold, int nsteps, double dtau) {
     if (!c) return TCYM_E_ARG;
     int r;
-    if ((r = tcym_copy_links(c, u_cold, u_hot))) return r;
     // the flowed E field of Cooling::DynamicFields is discarded by CoolNSave, so
-    // the fused flow keeps the force in registers (GRADIENTFLOW/Cooling.cpp:156-179)
+    // the fused flow keeps the force in registers (GRADIENTFLOW/Cooling.cpp:156-179).
+    // The first flow step reads the hot links in place of a copy of them.
+    if (nsteps > 0 && u_hot != u_cold)
+        return run_update(c, 3, u_cold, -1, -1, nsteps, dtau, 0.0, 0.0, 0, 0, false, u_hot);
+    if ((r = tcym_copy_links(c, u_cold, u_hot))) return r;
     return run_update(c, 3, u_cold, -1, -1, nsteps, dtau, 0.0, 0.0, 0, 0, false);
 }
 int tcym_reunitarize(tcym_ctx* c, int u) {

@@ -205,11 +205,13 @@ struct tcym_batch : tcym::DeviceQueue {
         drop_graphs();
         return sync();
     }
-    // one update step of every lattice: U[u] -> Uscr, then swap; `complete`
-    // false for a step whose links only the next update step reads
-    void step(int mode, int u, int e, bool write_e, double dt, double damping, double noise_sd, uint32_t step_off, bool complete) {
-        be->update_batch(L(), dev(), mode, write_e, dt, damping, noise_sd, step_off, U[u], Uscr, e >= 0 ? E[e] : nullptr, nullptr,
-                         complete);
+    // one update step of every lattice: U[u] (or U[src]) -> Uscr, then swap
+    // with U[u]; `complete` false for a step whose links only the next update
+    // step reads
+    void step(int mode, int u, int e, bool write_e, double dt, double damping, double noise_sd, uint32_t step_off, bool complete,
+              int src = -1) {
+        be->update_batch(L(), dev(), mode, write_e, dt, damping, noise_sd, step_off, src >= 0 ? U[src] : U[u], Uscr,
+                         e >= 0 ? E[e] : nullptr, nullptr, complete);
         std::swap(U[u], Uscr);
     }
     void advance(uint32_t steps) {
@@ -249,10 +251,14 @@ struct tcym_batch : tcym::DeviceQueue {
         be->copy(L(), u_elems, U[src], U[dst]);
         return status();
     }
-    int flow(int u, int nsteps, double dtau) {
+    // nsteps flow steps of slot u; with src >= 0 the first reads slot src
+    // (unchanged) in place of a copy of it in u
+    int flow(int u, int nsteps, double dtau, int src = -1) {
         int r;
         if ((r = useU(u)) || (r = scratch())) return r;
-        for (int k = 0; k < nsteps; k++) step(3, u, -1, false, dtau, 0.0, 0.0, 0, k == nsteps - 1);
+        if (src >= 0 && (r = useU(src))) return r;
+        if (src >= 0 && nsteps <= 0) return copy_links(u, src);
+        for (int k = 0; k < nsteps; k++) step(3, u, -1, false, dtau, 0.0, 0.0, 0, k == nsteps - 1, k == 0 ? src : -1);
         return status();
     }
 };
@@ -523,16 +529,14 @@ int tcym_batch_evolve(tcym_batch* b, int mode, int nseg, int steps, double dt, d
 int tcym_batch_cooling_start(tcym_batch* b, int nsteps, double dtau) {
     if (!b) return TCYM_E_ARG;
     int r;
-    if ((r = b->copy_links(TCYM_U_NEW, TCYM_U_HOT))) return r;
-    if ((r = b->flow(TCYM_U_NEW, nsteps, dtau))) return r;
+    if ((r = b->flow(TCYM_U_NEW, nsteps, dtau, TCYM_U_HOT))) return r;
     if ((r = b->copy_links(TCYM_U_OLD, TCYM_U_NEW))) return r;
     return b->sync();
 }
 int tcym_batch_cooling_update(tcym_batch* b, int nsteps, double dtau, double* dncs) {
     if (!b) return TCYM_E_ARG;
     int r;
-    if ((r = b->copy_links(TCYM_U_NEW, TCYM_U_HOT))) return r;
-    if ((r = b->flow(TCYM_U_NEW, nsteps, dtau))) return r;
+    if ((r = b->flow(TCYM_U_NEW, nsteps, dtau, TCYM_U_HOT))) return r;
     if ((r = b->useU(TCYM_U_OLD)) || (r = b->useU(TCYM_U_MID)) || (r = b->useE(TCYM_E_MID))) return r;
     b->be->geodesic_batch(b->L(), b->dev(), 0.5, b->U[TCYM_U_OLD], b->U[TCYM_U_NEW], b->U[TCYM_U_MID], b->E[TCYM_E_MID]);
     // CoolingMethod::GetDeltaNCs(1): Simpson (1,4,1)/6 (TOPOLOGY/CoolingMethod.cpp:80-121),
