@@ -1,0 +1,108 @@
+"""The FP32-mixed form against the FP64 form at the full sizes of the
+reference's parity configurations, through properties that do not need the
+oracle at that size:
+
+- SU(2) 64^3: links exp(i eps^a sigma^a/2), eps ~ N(0, 0.2^2), E ~ N(0, 0.3^2)
+  Gauss-projected; 1000 leapfrog steps dt = 0.05, then 1000 Langevin steps
+  (damping 0.1, T = 0.5);
+- SU(3) 96^3: eps ~ N(0, 0.15^2), E ~ N(0, 0.25^2) Gauss-projected; 500 leapfrog
+  steps dt = 0.04, then 500 gradient-flow steps dtau = 0.02 on a copy.
+
+Checked: Gauss law and unitarity conserved to rounding in FP64 and their
+growth in FP32, relative energy difference FP32 vs FP64 <= 1e-5, the
+hot-trajectory Delta N_CS (trapezoid of NCsDot over every step) within 1e-3,
+equipartition <E^a E^a> = n_E T per link after the Langevin steps, the
+magnetic energy falling monotonically under the flow, and the cooled fields'
+magnetic energy and vacuum N_CS estimator agreeing between the forms.  The
+Langevin N_CS integral is not compared: the two forms draw the same normals
+to their own precision, and over 1000 steps the noisy trajectories separate.
+The FP64 form itself is checked against the oracle in test_gpu_parity.py."""
+import numpy as np
+import pytest
+
+import topologicalcymsolver_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def initial_condition(nc, N, seed, sigma_u, sigma_e):
+    g = T.Solver(nc, N, T.FP64)
+    g.random_links(seed, sigma_u)
+    g.random_efield(seed, sigma_e)
+    it, fm = g.gauss_law_restore(maxiter=10000, tol=1e-14)
+    assert fm <= 1e-14, (it, fm)
+    U, E = g.get_links(), g.get_efield()
+    g.close()
+    return U, E
+
+
+def solver(nc, N, prec, U, E):
+    g = T.Solver(nc, N, prec)
+    g.set_links(U)
+    g.set_efield(E)
+    return g
+
+
+def test_su2_64_leapfrog_then_langevin():
+    N = (64, 64, 64)
+    U, E = initial_condition(2, N, 2002, 0.2, 0.3)
+    run = {}
+    for prec in (T.FP64, T.FP32):
+        g = solver(2, N, prec, U, E)
+        energy, ncs = [g.energy()], [0.0]
+        for _ in range(10):
+            ncs.append(ncs[-1] + g.evolve_ncs(0, 100, 0.05))
+            energy.append(g.energy())
+        gauss, unit = g.gauss_violation()[0], g.unitarity_violation()[0]
+        lncs = 0.0
+        for k in range(10):
+            lncs += g.evolve_ncs(1, 100, 0.05, damping=0.1, T=0.5, seed=77, step0=100 * k)
+        bulk = g.bulk_update()
+        run[prec] = dict(energy=np.array(energy), ncs=np.array(ncs), gauss=abs(gauss), unit=unit, esq=bulk[:3], lncs=lncs)
+        g.close()
+    d, f = run[T.FP64], run[T.FP32]
+    # FP64: Gauss law and unitarity conserved by the leapfrog to rounding;
+    # the energy oscillates at O(dt^2) without drift
+    assert d["gauss"] < 1e-12 and d["unit"] < 1e-12, (d["gauss"], d["unit"])
+    assert np.ptp(d["energy"]) < 1e-2 * d["energy"][0]
+    # FP32: growth of the violations over 1000 steps (FP64 reunitarisation each step)
+    assert f["gauss"] < 5e-5 and f["unit"] < 1e-6, (f["gauss"], f["unit"])
+    assert np.abs(f["energy"] - d["energy"]).max() <= 1e-5 * np.abs(d["energy"]).max()
+    assert np.abs(f["ncs"] - d["ncs"]).max() <= 1e-3, (f["ncs"][-1], d["ncs"][-1])
+    # Langevin at T = 0.5 after 50 time units (5 damping times): <E^a E^a> = n_E T = 1.5 per link
+    for r in (d, f):
+        assert np.all(np.abs(r["esq"] - 1.5) < 0.03 * 1.5), r["esq"]
+        assert np.isfinite(r["lncs"])
+    assert np.abs(f["esq"] - d["esq"]).max() < 2e-3 * 1.5
+
+
+def test_su3_96_leapfrog_then_cooling():
+    N = (96, 96, 96)
+    U, E = initial_condition(3, N, 3003, 0.15, 0.25)
+    run = {}
+    for prec in (T.FP64, T.FP32):
+        g = solver(3, N, prec, U, E)
+        energy, ncs = [g.energy()], [0.0]
+        for _ in range(5):
+            ncs.append(ncs[-1] + g.evolve_ncs(0, 100, 0.04))
+            energy.append(g.energy())
+        gauss, unit = g.gauss_violation()[0], g.unitarity_violation()[0]
+        g.copy_links(T.U_COOL, T.U_HOT)
+        bmag = [g.bulk_update(u=T.U_COOL, e=T.E_COOL)[3:].sum()]
+        for _ in range(5):
+            g.gradient_flow(100, 0.02, u=T.U_COOL, e=T.E_COOL)
+            bmag.append(g.bulk_update(u=T.U_COOL, e=T.E_COOL)[3:].sum())
+        run[prec] = dict(energy=np.array(energy), ncs=np.array(ncs), gauss=abs(gauss), unit=unit, bmag=np.array(bmag),
+                         vac=g.vacuum_ncs(u=T.U_COOL))
+        g.close()
+    d, f = run[T.FP64], run[T.FP32]
+    assert d["gauss"] < 1e-12 and d["unit"] < 1e-12, (d["gauss"], d["unit"])
+    assert np.ptp(d["energy"]) < 1e-2 * d["energy"][0]
+    assert f["gauss"] < 5e-5 and f["unit"] < 1e-6, (f["gauss"], f["unit"])
+    assert np.abs(f["energy"] - d["energy"]).max() <= 1e-5 * np.abs(d["energy"]).max()
+    assert np.abs(f["ncs"] - d["ncs"]).max() <= 1e-3, (f["ncs"][-1], d["ncs"][-1])
+    # the flow lowers the magnetic energy monotonically, in both forms alike
+    for r in (d, f):
+        assert np.all(np.diff(r["bmag"]) < 0), r["bmag"]
+    assert np.abs(f["bmag"] - d["bmag"]).max() <= 1e-4 * d["bmag"].max()
+    assert abs(f["vac"] - d["vac"]) <= 1e-3, (f["vac"], d["vac"])
