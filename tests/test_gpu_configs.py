@@ -6,7 +6,8 @@ oracle at that size:
   Gauss-projected; 1000 leapfrog steps dt = 0.05, then 1000 Langevin steps
   (damping 0.1, T = 0.5);
 - SU(3) 96^3: eps ~ N(0, 0.15^2), E ~ N(0, 0.25^2) Gauss-projected; 500 leapfrog
-  steps dt = 0.04, then 500 gradient-flow steps dtau = 0.02 on a copy.
+  steps dt = 0.04, then 500 gradient-flow steps dtau = 0.02 on a copy;
+- 64 batched SU(2) lattices, and SU(3) 192^3 leapfrog with cooling (below).
 
 Checked: Gauss law and unitarity conserved to rounding in FP64 and their
 growth in FP32, relative energy difference FP32 vs FP64 <= 1e-5, the
@@ -133,3 +134,31 @@ def test_batched_su2_64_ensembles():
     assert d["unit"] < 1e-12 and f["unit"] < 1e-6, (d["unit"], f["unit"])
     assert np.abs(f["dncs"] - d["dncs"]).max() <= 1e-3, np.abs(f["dncs"] - d["dncs"]).max()
     assert np.abs(f["ncsdot"] - d["ncsdot"]).max() <= 1e-3, np.abs(f["ncsdot"] - d["ncsdot"]).max()
+
+
+def test_su3_192_leapfrog_with_cooling():
+    """SU(3) 192^3 (eps ~ N(0, 0.15^2), E ~ N(0, 0.25^2) Gauss-projected): 1000
+    leapfrog steps dt = 0.04, every 50 steps 200 flow steps dtau = 0.02 on a
+    copy and the geodesic Delta N_CS between consecutive cooled copies.  FP32
+    against FP64 over the run: |Delta N_CS| <= 1e-3 and relative energy
+    difference <= 1e-5 at every measurement."""
+    N = (192, 192, 192)
+    U, E = initial_condition(3, N, 5005, 0.15, 0.25)
+    run = {}
+    for prec in (T.FP64, T.FP32):
+        g = solver(3, N, prec, U, E)
+        g.cooling_method_start(200, 0.02)
+        energy, ncs, total = [], [], 0.0
+        for _ in range(20):
+            g.dynamics_update(50, 0.04)
+            total += g.cooling_method_update(200, 0.02)
+            ncs.append(total)
+            energy.append(g.energy())
+        run[prec] = dict(energy=np.array(energy), ncs=np.array(ncs), gauss=abs(g.gauss_violation()[0]),
+                         unit=g.unitarity_violation()[0])
+        g.close()
+    d, f = run[T.FP64], run[T.FP32]
+    assert d["gauss"] < 1e-12 and d["unit"] < 1e-12, (d["gauss"], d["unit"])
+    assert f["gauss"] < 5e-5 and f["unit"] < 1e-6, (f["gauss"], f["unit"])
+    assert np.abs(f["ncs"] - d["ncs"]).max() <= 1e-3, np.abs(f["ncs"] - d["ncs"]).max()
+    assert np.abs(f["energy"] - d["energy"]).max() <= 1e-5 * np.abs(d["energy"]).max()
