@@ -100,7 +100,12 @@ int tcym_unit_links(tcym_ctx *ctx, int slot);                       /* GaugeLink
 int tcym_random_links(tcym_ctx *ctx, int slot, uint64_t seed, double sigma);
 int tcym_random_efield(tcym_ctx *ctx, int slot, uint64_t seed, double sigma);
 
-/* ---- updates ---- */
+/* ---- updates ----
+   An update call of nsteps steps is faster than nsteps calls of one step:
+   between its steps the SU(3) kernels keep only columns 0 and 1 of the
+   links (the next step rebuilds column 2), and the host synchronises once.
+   The fields are bit-identical either way, and the links a call leaves in
+   its slot are complete. */
 /* Dynamics::Update x nsteps with dTau = dt (DYNAMICS/Dynamics.cpp:204-218) */
 int tcym_leapfrog(tcym_ctx *ctx, int u, int e, int nsteps, double dt);
 /* Dynamics::UpdateElectricFields / UpdateGaugeLinks on the full box (DYNAMICS/Dynamics.cpp:145,97) */
