@@ -133,6 +133,52 @@ def test_observables_match_oracle(nc, N):
 
 
 @pytest.mark.parametrize("nc,N", CASES)
+def test_anisotropic_lattice_matches_oracle(nc, N):
+    """Spacings (1, 1.3, 0.7) and aScale 0.9 enter every coefficient (the
+    oracle's anisotropic force and energy are pinned in test_oracle_lattice.py):
+    leapfrog, Langevin, flow, the observables and GaussRestore."""
+    a, asc = (1.0, 1.3, 0.7), 0.9
+
+    def make(seed, su, se, restore=False):
+        lat = O.Lattice(nc, N, a=a, aScale=asc)
+        lat.random(seed, su, se, restore=restore)
+        gpu = T.Solver(nc, N, T.FP64, a=a, aScale=asc)
+        gpu.set_links(lat.U)
+        gpu.set_efield(lat.E)
+        return lat, gpu
+
+    lat, gpu = make(41, 0.5, 0.4)
+    lat.leapfrog(6, 0.05)
+    gpu.dynamics_update(6, 0.05)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-12)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-12)
+    lat.langevin(4, 0.05, 0.2, 0.5, seed=3, step0=2)
+    gpu.langevin_update(4, 0.05, 0.2, 0.5, seed=3, step0=2)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-12)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-12)
+    for improved in (True, False):
+        want = lat.edotb() if improved else lat.edotb_unimproved()
+        assert abs(gpu.edotb(improved) - want) < 1e-12 * max(1.0, abs(want)), improved
+    np.testing.assert_allclose(gpu.edotb_map(), lat.edotb_map(), atol=1e-12)
+    assert abs(gpu.ncs_dot() - lat.ncsdot()) < 1e-13
+    np.testing.assert_allclose(gpu.bulk_update(), lat.bulk(), rtol=1e-13, atol=1e-14)
+    gm, gr = gpu.gauss_violation()
+    om, orr = lat.gauss()
+    assert abs(gm - om) < 1e-12 and abs(gr - orr) < 1e-12
+    assert abs(gpu.vacuum_ncs() - lat.vacuum_ncs()) < 1e-13
+    lat, gpu = make(42, 0.6, 0.0)
+    lat.flow(4, 0.03)
+    gpu.gradient_flow(4, 0.03, u=T.U_HOT, e=T.E_HOT)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-12)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-12)
+    lat, gpu = make(43, 0.5, 0.5)
+    it_o, _ = lat.gauss_restore()
+    it_g, _ = gpu.gauss_law_restore()
+    assert it_g == it_o, (it_g, it_o)
+    np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-12)
+
+
+@pytest.mark.parametrize("nc,N", CASES)
 def test_gauss_restore_matches_oracle(nc, N):
     lat, gpu = pair(nc, N, 14, 0.5, 0.5)
     it_o, fm_o = lat.gauss_restore()

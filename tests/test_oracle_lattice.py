@@ -96,6 +96,70 @@ def test_leapfrog_energy_drift_is_second_order(nc):
     assert 3.2 < ratio < 4.8, (drifts, ratio)
 
 
+# anisotropic spacings: the Hamiltonian of the reference's update coefficients
+# (E += -2 dt a^3 cE_mu cE_nu [...], cE = 1/a^2; U <- exp(-i dt a_mu^2/a^3 E) U) is
+#   H = sum_x sum_mu a_mu^2/(2 a^3) (E_mu^2 + B_mu^2),  B_mu^2 = 4 ReTr(1 - P_(nu rho)),
+# the plaquette of plane (nu, rho) weighted 2 a^3/(a_nu^2 a_rho^2); a wrong power
+# of a in the force, the link update or this energy fails one of the two tests below
+ANISO = (1.0, 1.3, 0.7)
+
+
+def aniso_weights(lat):
+    a = np.array(lat.p.a[:3])
+    return a * a / (2.0 * a.prod())
+
+
+def aniso_magnetic_energy(lat):
+    return lat.V * float(np.dot(aniso_weights(lat), lat.bulk()[3:]))
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_force_is_gradient_of_magnetic_energy_anisotropic(nc):
+    lat = O.Lattice(nc, 3, a=ANISO)
+    lat.random(12, 0.6, 0.0, restore=False)
+    lat.E[:] = 0
+    dt = 1e-3
+    ref = lat.copy()
+    lat.leapfrog(1, dt)
+    # dE/dt = dH_B/dalpha with dalpha/dt = -dH/dE = -(a_mu^2/a^3) E
+    rate = lat.E.reshape(-1, O.ne(nc)) / dt
+    rng = np.random.default_rng(6)
+    eps = 1e-5
+    for l in rng.choice(3 * lat.V, size=6, replace=False):
+        for a in range(O.ne(nc)):
+            vals = []
+            for sgn in (+1, -1):
+                t = ref.copy()
+                al = np.zeros(O.ne(nc))
+                al[a] = sgn * eps
+                u = t.U.reshape(-1, O.su(nc))
+                u[l] = O.mul(nc, O.expi(nc, 1.0, al), u[l])
+                vals.append(aniso_magnetic_energy(t))
+            grad = (vals[0] - vals[1]) / (2 * eps)
+            assert abs(rate[l, a] - grad) < 2e-8 * max(1.0, abs(grad)), (l, a, rate[l, a], grad)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_leapfrog_energy_drift_is_second_order_anisotropic(nc):
+    lat0 = O.Lattice(nc, 4, a=ANISO)
+    lat0.random(22, 0.5, 0.4)
+    w = np.repeat(aniso_weights(lat0), O.ne(nc))          # per (mu, a) of one site
+    drifts = []
+    for dt in (0.04, 0.02):
+        lat = lat0.copy()
+        dev, H0 = 0.0, None
+        for _ in range(int(round(1.6 / dt))):
+            Eprev, Uprev = lat.E.copy(), lat.copy()
+            lat.leapfrog(1, dt)
+            Esync = (0.5 * (Eprev + lat.E)).reshape(lat.V, 3 * O.ne(nc))
+            H = float((Esync * Esync * w).sum()) + aniso_magnetic_energy(Uprev)
+            H0 = H if H0 is None else H0
+            dev = max(dev, abs(H - H0))
+        drifts.append(dev)
+    ratio = drifts[0] / drifts[1]
+    assert 3.2 < ratio < 4.8, (drifts, ratio)
+
+
 @pytest.mark.parametrize("nc", [2, 3])
 def test_leapfrog_conserves_gauss_law(nc):
     lat = O.Lattice(nc, 4)
