@@ -20,18 +20,18 @@ SEED0 = 4004
 NOISE0 = 910
 
 
-def build(nc, sizes, prec, su=0.2, se=0.3):
+def build(nc, sizes, prec, su=0.2, se=0.3, a=(1.0, 1.0, 1.0), aScale=1.0):
     """A batch and one Solver per lattice with the same initial condition."""
-    b = T.Batch(nc, sizes, prec)
+    b = T.Batch(nc, sizes, prec, a=a, aScale=aScale)
     iters = b.initial_condition(SEED0, su, se, maxiter=3000, tol=1e-14)
     singles = []
     for e, n in enumerate(sizes):
-        g64 = T.Solver(nc, n, T.FP64)
+        g64 = T.Solver(nc, n, T.FP64, a=a, aScale=aScale)
         g64.random_links(SEED0 + e, su)
         g64.random_efield(SEED0 + e, se)
         it, _ = g64.gauss_law_restore(maxiter=3000, tol=1e-14)
         assert it == iters[e]
-        s = T.Solver(nc, n, prec)
+        s = T.Solver(nc, n, prec, a=a, aScale=aScale)
         s.set_links(g64.get_links())
         s.set_efield(g64.get_efield())
         g64.close()
@@ -60,8 +60,9 @@ def assert_same_measurements(b, singles):
 
 @pytest.mark.parametrize("nc,sizes", [(2, SU2_SIZES), (3, SU3_SIZES)])
 @pytest.mark.parametrize("prec", [T.FP32, T.FP64])
-def test_batch_matches_single_lattices(nc, sizes, prec):
-    b, singles = build(nc, sizes, prec)
+@pytest.mark.parametrize("spacing", [((1.0, 1.0, 1.0), 1.0), ((1.0, 1.3, 0.7), 0.9)])
+def test_batch_matches_single_lattices(nc, sizes, prec, spacing):
+    b, singles = build(nc, sizes, prec, a=spacing[0], aScale=spacing[1])
     assert_same_fields(b, singles)
     assert_same_measurements(b, singles)
     # Langevin: key NOISE0 + e, Philox steps continue across calls
