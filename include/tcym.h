@@ -204,6 +204,17 @@ int tcym_reunitarize(tcym_ctx *ctx, int u);
 int tcym_edotb(tcym_ctx *ctx, int u, int e, int improved, double *out);
 /* ChernSimonsNumber::NCsDot = E.B/(8 pi^2) (TOPOLOGY/ChernSimonsDerivative.cpp:31-49) */
 int tcym_ncsdot(tcym_ctx *ctx, int u, int e, double *out);
+/* tcym_ncsdot without any allocation.  No counterpart in the reference: it is
+   for callers that cannot allocate device memory (code under HIP graph
+   capture, serving loops, harnesses that forbid hipMalloc).  out is DEVICE
+   memory of one double, written asynchronously on the context's stream
+   (tcym_sync waits); the value is bit-identical to tcym_ncsdot's.  Slots u and
+   e must already hold fields (this call never allocates a slot).  The block
+   partials live in a fixed 1 MiB device array per GPU (131072 partial sums:
+   lattices up to about 220^3); larger lattices are refused with TCYM_E_ARG.
+   Every call on a GPU shares that array: calls that overlap on one GPU (two
+   contexts on two host threads) race. */
+int tcym_ncsdot_noalloc(tcym_ctx *ctx, int u, int e, double *out);
 /* Observables::EdotBMap::CreateMap: per-site improved E.B, map[site] (OBSERVABLES/EdotBMap.cpp:20-348) */
 int tcym_edotb_map(tcym_ctx *ctx, int u, int e, double *map);
 /* Observables::Bulk::Update: out = {ESqr[3], BSqr[3]} (OBSERVABLES/BulkObservables.cpp:133-143) */

@@ -29,7 +29,7 @@ EXPORTS = [
     "tcym_random_links", "tcym_random_efield",
     "tcym_leapfrog", "tcym_update_efield", "tcym_update_links", "tcym_langevin", "tcym_langevin_xi", "tcym_evolve_ncs",
     "tcym_noise", "tcym_flow", "tcym_cool", "tcym_cooling_start", "tcym_cooling_update", "tcym_reunitarize",
-    "tcym_edotb", "tcym_ncsdot", "tcym_edotb_map", "tcym_bulk", "tcym_hard_scales", "tcym_gauss", "tcym_unitarity",
+    "tcym_edotb", "tcym_ncsdot", "tcym_ncsdot_noalloc", "tcym_edotb_map", "tcym_bulk", "tcym_hard_scales", "tcym_gauss", "tcym_unitarity",
     "tcym_gauss_restore", "tcym_geodesic", "tcym_delta_ncs_cooling", "tcym_vacuum_ncs",
     "tcym_block", "tcym_level", "tcym_cool_blocked", "tcym_continue_cooling", "tcym_thermalize",
     "tcym_quasi_particles",
@@ -110,7 +110,7 @@ def load_library(path=LIB_PATH):
         "tcym_noise": [V, I, U64, U32, D], "tcym_flow": [V, I, I, I, D, I, _dp],
         "tcym_cool": [V, I, I, I, D], "tcym_cooling_start": [V, I, I, D],
         "tcym_cooling_update": [V, I, I, D, I, _dp], "tcym_reunitarize": [V, I],
-        "tcym_edotb": [V, I, I, I, _dp], "tcym_ncsdot": [V, I, I, _dp], "tcym_edotb_map": [V, I, I, _dp],
+        "tcym_edotb": [V, I, I, I, _dp], "tcym_ncsdot": [V, I, I, _dp], "tcym_ncsdot_noalloc": [V, I, I, V], "tcym_edotb_map": [V, I, I, _dp],
         "tcym_bulk": [V, I, I, _dp], "tcym_gauss": [V, I, I, _dp, _dp], "tcym_unitarity": [V, I, _dp, _dp],
         "tcym_hard_scales": [V, I, _dp],
         "tcym_gauss_restore": [V, I, I, I, D, ctypes.POINTER(I), _dp],
@@ -419,6 +419,11 @@ class Solver:
         d = ctypes.c_double()
         self._ck(self.L.tcym_ncsdot(self.h, u, e, ctypes.byref(d)))
         return d.value
+
+    def ncs_dot_noalloc(self, out_dev, u=U_HOT, e=E_HOT):
+        """tcym_ncsdot_noalloc: NCsDot written to the device address out_dev (an
+        int), asynchronously on the context's stream; call sync() before reading it."""
+        self._ck(self.L.tcym_ncsdot_noalloc(self.h, u, e, ctypes.c_void_p(out_dev)))
 
     def edotb_map(self, u=U_HOT, e=E_HOT):
         m = np.empty(self.V)

@@ -49,6 +49,41 @@ __global__ void k_hot_ncs(const double* __restrict__ eb, double* __restrict__ ac
     }
 }
 
+// ---- scratch of tcym_ncsdot_noalloc ----
+// The block partials of the improved E.B live in this fixed device array:
+// the entry point allocates nothing.  Each GPU holds its own copy at its own
+// address, looked up once per device (scratch_ptr) while tcym_create has that
+// device current; host code uses only that address, never the array's name.
+// Calls that overlap on one GPU share the array and race.
+constexpr long kScratchDoubles = 131072;   // 1 MiB: the partials of lattices up to about 220^3
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// the device address of `scratch` on `device` (the current device), looked up on first use
+hipError_t scratch_ptr(int device, double** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = (double*)p;
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+
+// final reduction of the E.B partials, written as NCsDot = E.B / (8 pi^2)
+// (TOPOLOGY/ChernSimonsDerivative.cpp:47); same sum order and rounding as
+// k_final followed by the host division of tcym_ncsdot
+__global__ __launch_bounds__(256) void k_final_ncsdot(const double* __restrict__ part, long nblk, double* __restrict__ out) {
+    const double r = tcym::final_reduce_block(part, nblk, 1, 0, false);
+    if (threadIdx.x == 0) out[0] = 0.125*r/(M_PI*M_PI);
+}
+
 }  // namespace
 
 namespace tcym {
@@ -189,6 +224,8 @@ int tcym_create(const tcym_params* p, tcym_ctx** out) {
     c->be = p->nc == 2 ? tcym::make_backend_su2(f32) : tcym::make_backend_su3(f32);
     c->be64 = f32 ? (p->nc == 2 ? tcym::make_backend_su2(0) : tcym::make_backend_su3(0)) : c->be;
     int r = c->hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    double* scr = nullptr;
+    if (!r) r = c->hip(scratch_ptr(p->device, &scr), "hipGetSymbolAddress(scratch)");
     long nblk = 0;
     for (int lev = 0; lev < 8 && c->geo(lev).n[0] >= 1 && c->geo(lev).n[1] >= 1 && c->geo(lev).n[2] >= 1; lev++)
         nblk = std::max(nblk, tcym::walk_shape(c->geo(lev)).blocks);
@@ -480,6 +517,26 @@ int tcym_edotb(tcym_ctx* c, int u, int e, int improved, double* out) {
 int tcym_ncsdot(tcym_ctx* c, int u, int e, double* out) {
     if (!c || !out) return TCYM_E_ARG;
     return c->ncsdot(u, e, out);
+}
+int tcym_ncsdot_noalloc(tcym_ctx* c, int u, int e, double* out) {
+    if (!c || !out) return TCYM_E_ARG;
+    if (u < 0 || u >= TCYM_NUM_U || e < 0 || e >= TCYM_NUM_E) return c->fail(TCYM_E_ARG, "slot out of range");
+    if (c->last != hipSuccess) return TCYM_E_HIP;
+    int r;
+    Geom q;
+    if ((r = c->at(q, u, e))) return r;
+    const long nblk = 3*tcym::walk_shape(q).blocks;    // partials of the improved E.B (one per block)
+    if (nblk > kScratchDoubles)
+        return c->fail(TCYM_E_ARG, "tcym_ncsdot_noalloc: the lattice needs " + std::to_string(nblk) +
+                                       " partial sums, beyond the fixed scratch of " + std::to_string(kScratchDoubles));
+    if (!c->U[u] || !c->E[e])
+        return c->fail(TCYM_E_ARG, "tcym_ncsdot_noalloc: the slot holds no field (this entry point allocates no slot)");
+    double* scr = nullptr;
+    if ((r = c->hip(scratch_ptr(c->device, &scr), "hipGetSymbolAddress(scratch)"))) return r;
+    // launches go straight to the stream: no timing events, no host copy, no synchronisation
+    const long nb = c->be->edotb_on(c->stream, q, c->U[u], c->E[e], scr);
+    hipLaunchKernelGGL(k_final_ncsdot, dim3(1), dim3(256), 0, c->stream, (const double*)scr, nb, out);
+    return c->hip(hipGetLastError(), "tcym_ncsdot_noalloc launch");
 }
 int tcym_edotb_map(tcym_ctx* c, int u, int e, double* map) {
     if (!c || !map) return TCYM_E_ARG;

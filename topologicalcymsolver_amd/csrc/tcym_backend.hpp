@@ -81,6 +81,8 @@ struct Backend {
     virtual void link_update(const Launcher&, const Geom&, const double* cU, const void* Uin, void* Uout, const void* E) = 0;
     // reductions: per-block partials (double) written to `part`; return number of blocks
     virtual long edotb(const Launcher&, const Geom&, int improved, const void* U, const void* E, double* part, double* map) = 0;
+    // the improved E.B partials launched straight onto a stream (no launcher object, no allocation)
+    virtual long edotb_on(hipStream_t, const Geom&, const void* U, const void* E, double* part) = 0;
     virtual long bulk(const Launcher&, const Geom&, const void* U, const void* E, double* part) = 0;
     virtual long gauss(const Launcher&, const Geom&, const void* U, const void* E, double* part, double* G) = 0;
     virtual long unitarity(const Launcher&, const Geom&, const void* U, double* part) = 0;
