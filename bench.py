@@ -1,13 +1,18 @@
 """Benchmark of the real-time lattice update loop on one MI355X.
 
-Workload (one step = one leapfrog step of the hot lattice):
-  SU(3) 192^3, links exp(i eps^a lambda^a/2) with eps^a ~ N(0, 0.15^2) and
-  E^a ~ N(0, 0.25^2) Gauss-projected (seed 5005); leapfrog dt = 0.04; every
-  50 steps the hot links are cooled on a copy by 200 gradient-flow steps
-  (dtau = 0.02) and Delta N_CS between consecutive cooled configurations is
-  measured on their geodesic (Simpson 1-4-1 of the improved E.B).
-The FP32 form (FP32 storage and arithmetic, FP64 reunitarisation and FP64
-sums) is timed first, then the FP64 form on the same initial condition.
+Workload: SU(3) 192^3, links exp(i eps^a lambda^a/2) with eps^a ~ N(0, 0.15^2)
+and E^a ~ N(0, 0.25^2) Gauss-projected (seed 5005); leapfrog dt = 0.04; every
+50 leapfrog steps the hot links are cooled on a copy by 200 gradient-flow
+steps (dtau = 0.02) and Delta N_CS between consecutive cooled configurations
+is measured on their geodesic (Simpson 1-4-1 of the improved E.B).
+
+One bench step is one such measurement interval: 50 leapfrog steps, one
+200-step cooling episode and the geodesic Delta N_CS.  The timed job is the
+initial cooling (CoolingMethod::Start) and --steps intervals, so the default
+20 steps are the whole 1000-step trajectory: 1000 leapfrog and 4200 flow link
+updates per link.  The FP32 form (FP32 storage and arithmetic, FP64
+reunitarisation and FP64 sums) is timed first, then the FP64 form on the same
+initial condition.
 
 value = link updates per second of the FP32 form over the whole timed job
 (leapfrog + flow link updates / wall time).  One JSON line on stdout.
@@ -86,16 +91,20 @@ def make_initial_condition(T, nc, N, seed, su, se, maxiter):
 
 
 def run_form(T, nc, N, precision, U, E, steps, warmup, dt, cool_every, cool_steps, dtau, timing):
+    """`steps` measurement intervals (cool_every leapfrog steps, a cool_steps
+    cooling episode, the geodesic Delta N_CS and the energy) after the initial
+    cooling, all timed; `warmup` untimed intervals run first on copies."""
     g = T.Solver(nc, N, precision)
     g.set_links(U)
     g.set_efield(E)
-    # warm-up on copies of the fields: leapfrog and a short cooling episode
+    # warm-up on copies of the fields: the same calls as a measurement interval
     g.copy_links(T.U_AUX, T.U_HOT)
     g.copy_efield(T.E_AUX, T.E_HOT)
-    if warmup > 0:
-        g.dynamics_update(warmup, dt, u=T.U_AUX, e=T.E_AUX)
-        g.cool_n_save(min(warmup, cool_steps), dtau, u_hot=T.U_AUX, u_cold=T.U_COOL)
+    for _ in range(warmup):
+        g.dynamics_update(cool_every, dt, u=T.U_AUX, e=T.E_AUX)
+        g.cool_n_save(cool_steps, dtau, u_hot=T.U_AUX, u_cold=T.U_COOL)
         g.ncs_dot(u=T.U_AUX, e=T.E_AUX)
+        g.energy(u=T.U_AUX, e=T.E_AUX)
     g.sync()
     if timing:
         g.timing(True)
@@ -107,21 +116,18 @@ def run_form(T, nc, N, precision, U, E, steps, warmup, dt, cool_every, cool_step
     t0 = time.perf_counter()
     g.cooling_method_start(cool_steps, dtau)
     n_flow += cool_steps
-    k = 0
-    while k < steps:
-        # the leapfrog steps up to the next measurement in one call
-        n = min(cool_every - k % cool_every, steps - k)
-        g.dynamics_update(n, dt)
-        n_lf += n
-        k += n
-        if k % cool_every == 0 or k == steps:
-            d = g.cooling_method_update(cool_steps, dtau, measure=True)
-            n_flow += cool_steps
-            ncs += d
-            dncs_series.append(ncs)
-            energies.append(g.energy())
-            if k % (4*cool_every) == 0:
-                log("#   step %d: T00 %.10e  Delta N_CS %.6e  (%.1f s)" % (k, energies[-1], ncs, time.perf_counter() - t0))
+    for k in range(1, steps + 1):
+        # the leapfrog steps of one interval in one call, then the measurement
+        g.dynamics_update(cool_every, dt)
+        n_lf += cool_every
+        d = g.cooling_method_update(cool_steps, dtau, measure=True)
+        n_flow += cool_steps
+        ncs += d
+        dncs_series.append(ncs)
+        energies.append(g.energy())
+        if k % 4 == 0:
+            log("#   interval %d (leapfrog step %d): T00 %.10e  Delta N_CS %.6e  (%.1f s)"
+                % (k, n_lf, energies[-1], ncs, time.perf_counter() - t0))
     g.sync()
     wall = time.perf_counter() - t0
     stats = {}
@@ -198,12 +204,20 @@ def run_batched(T, count, seed, precision, lang_steps, seg, cool_every, cool_ste
                 unitarity=float(un[:, 0].max()), gauss=float(np.abs(ga[:, 0]).max()))
 
 
-def cpu_baseline(nc, N, U, E, dt, dtau, budget_s):
-    """The oracle (FP64 CPU restatement) on the host cores: leapfrog and flow
-    steps on the same lattice, until about budget_s seconds are spent."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import _oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_sample(O, nc, N, U, E, dt, dtau, budget_s, threads):
+    """leapfrog then flow steps of the oracle, each kind until about budget_s/2 s
+    are spent; the rate weights them as the timed job does (1000 : 4200)"""
     lat = O.Lattice(nc, N, nthreads=threads)
     lat.U[:] = U
     lat.E[:] = E
@@ -222,18 +236,35 @@ def cpu_baseline(nc, N, U, E, dt, dtau, budget_s):
             break
     t2 = time.perf_counter()
     V = N[0] * N[1] * N[2]
-    lu = 3 * V * (n_lf + n_fl)
-    return dict(value=lu / (t2 - t0), unit="link-updates/s", cores=threads, kind="port",
-                sample="%d leapfrog + %d flow steps of the oracle on SU(%d) %dx%dx%d (%.1f s, FP64, OpenMP)"
-                       % (n_lf, n_fl, nc, N[0], N[1], N[2], t2 - t0),
-                leapfrog_lu_s=3 * V * n_lf / (t1 - t0), flow_lu_s=3 * V * n_fl / (t2 - t1))
+    t_lf, t_fl = (t1 - t0) / n_lf, (t2 - t1) / n_fl
+    return dict(n_lf=n_lf, n_fl=n_fl, wall=t2 - t0, lf=3 * V / t_lf, fl=3 * V / t_fl,
+                mix=3 * V * 5200 / (1000 * t_lf + 4200 * t_fl))
+
+
+def cpu_baseline(nc, N, U, E, dt, dtau, budget_s):
+    """The oracle (FP64 CPU restatement) on the host cores, and on one core:
+    leapfrog and flow steps on the same lattice for about budget_s seconds
+    each run; the rate is that of the job's 1000 : 4200 leapfrog : flow mix."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    a = cpu_sample(O, nc, N, U, E, dt, dtau, budget_s, threads)
+    one = cpu_sample(O, nc, N, U, E, dt, dtau, budget_s, 1)
+    return dict(value=a["mix"], unit="link-updates/s", cores=threads, kind="port", cpu=cpu_model(),
+                sample="%d leapfrog + %d flow steps of the oracle on SU(%d) %dx%dx%d (%.1f s, FP64, OpenMP, "
+                       "%d threads), rate weighted 1000 : 4200 as the timed job"
+                       % (a["n_lf"], a["n_fl"], nc, N[0], N[1], N[2], a["wall"], threads),
+                leapfrog_lu_s=a["lf"], flow_lu_s=a["fl"],
+                single_thread={"value": one["mix"], "leapfrog_lu_s": one["lf"], "flow_lu_s": one["fl"],
+                               "sample": "%d leapfrog + %d flow steps on 1 thread (%.1f s)"
+                                         % (one["n_lf"], one["n_fl"], one["wall"])})
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20, help="measurement intervals (50 leapfrog steps + one cooling each)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed measurement intervals, on copies of the fields")
     ap.add_argument("--N", type=int, default=192)
     ap.add_argument("--nc", type=int, default=3)
     ap.add_argument("--dt", type=float, default=0.04)
@@ -243,7 +274,7 @@ def main():
     ap.add_argument("--seed", type=int, default=5005)
     ap.add_argument("--restore-maxiter", type=int, default=10000)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 form")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=16.0, help="CPU baseline budget per run (0 = skip)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensembles workload")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=500)
@@ -257,8 +288,8 @@ def main():
     import topologicalcymsolver_amd as T
     nc, N = args.nc, (args.N, args.N, args.N)
     V = N[0] * N[1] * N[2]
-    log("# SU(%d) %d^3, %d steps (+%d warm-up), cooling %d flow steps every %d" %
-        (nc, args.N, args.steps, args.warmup, args.cool_steps, args.cool_every))
+    log("# SU(%d) %d^3, %d intervals of %d leapfrog steps and a %d-step cooling (+%d warm-up intervals)" %
+        (nc, args.N, args.steps, args.cool_every, args.cool_steps, args.warmup))
     U, E, it, fm = make_initial_condition(T, nc, N, args.seed, 0.15, 0.25, args.restore_maxiter)
 
     r32 = run_form(T, nc, N, T.FP32, U, E, args.steps, args.warmup, args.dt, args.cool_every,
@@ -311,6 +342,8 @@ def main():
         "config": {
             "workload": "SU(%d) %d^3 FP32-mixed leapfrog dt=%g, %d-step gradient-flow cooling dtau=%g every %d steps, "
                         "geodesic Delta N_CS" % (nc, args.N, args.dt, args.cool_steps, args.dtau, args.cool_every),
+            "step": "one measurement interval: %d leapfrog steps + a %d-step cooling + geodesic Delta N_CS"
+                    % (args.cool_every, args.cool_steps),
             "leapfrog_steps": r32["n_lf"], "flow_steps": r32["n_flow"],
             "gauss_restore_iterations": it, "gauss_restore_signed_max": fm,
         },
@@ -319,6 +352,10 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_algorithmic": 3 * V * bpl, "traffic_source": traffic_src,
             "bytes_per_link_update": bpl, "avg_launch_us": avg_s * 1e6,
+            "frac_measured_traffic": (traffic / avg_s / 1e9 / HBM_PEAK_GBS) if traffic else None,
+            "bytes_basis": "algorithmic: every link read once and written once per step (2 s_U w = %d B); between the "
+                           "steps of one call the kernel moves columns 0 and 1 of the links only, so the measured "
+                           "traffic is lower" % bpl,
         },
         "kernels": kernels,
         "fp32": {"wall_s": r32["wall"], "link_updates_per_s": lu32, "unitarity_max": r32["unitarity"],
