@@ -249,13 +249,19 @@ static inline void su3_heig(const ocd *Ain, ocd *V, double *w) {
         double r = sqrt(p/3.0);
         double disc = 4.0*p*p*p/27.0 - q*q;
         double phi = atan2(sqrt(disc > 0.0 ? disc : 0.0), q)/3.0;
+        double scale = sqrt(p);
         for (int k = 0; k < 3; k++) {
             double l = 2.0*r*cos(phi - 2.0*M_PI*k/3.0);
-            double dp = 3.0*l*l - p;                        /* one Newton step */
-            if (dp != 0.0) l -= (l*l*l - p*l - q)/dp;
+            /* one Newton step, kept only as a refinement: at a (near) double
+               root the derivative 3 l^2 - p vanishes and the step would throw
+               l far off (and past the degeneracy test below) */
+            double dp = 3.0*l*l - p;
+            if (dp != 0.0) {
+                double st = (l*l*l - p*l - q)/dp;
+                if (fabs(st) < 1e-6*scale) l -= st;
+            }
             lam[k] = l;
         }
-        double scale = sqrt(p);
         double g01 = fabs(lam[0]-lam[1]), g02 = fabs(lam[0]-lam[2]), g12 = fabs(lam[1]-lam[2]);
         double gmin = g01 < g02 ? g01 : g02; if (g12 < gmin) gmin = g12;
         ok = gmin >= 1e-3*scale;

@@ -64,6 +64,26 @@ def test_su3_exp_degenerate_spectra():
         np.testing.assert_allclose(got, want, atol=3e-15)
 
 
+def test_su3_exp_degenerate_spectra_all_angles():
+    # doubly degenerate spectra at every angle: along lambda_8, along
+    # (lambda_3 +- sqrt(3) lambda_8)/2 and along random conjugates V lambda_8 V^+
+    # (a Newton step on the double root of the characteristic cubic once threw
+    # the eigenvalue off here)
+    t = O.generators(3)
+    rng = np.random.default_rng(13)
+    dirs = [np.eye(8)[7], np.array([0, 0, 0.5, 0, 0, 0, 0, 0.5 * np.sqrt(3)]),
+            np.array([0, 0, 0.5, 0, 0, 0, 0, -0.5 * np.sqrt(3)])]
+    for _ in range(4):
+        V = sla.expm(1j * O.alg_matrix(3, rng.normal(0, 1.0, 8)))
+        L = V @ (2 * t[7]) @ V.conj().T
+        dirs.append(np.array([2 * np.real(np.trace(ta @ L)) for ta in t]))
+    for d in dirs:
+        for th in np.concatenate([np.linspace(-3, 3, 601), rng.normal(0, 1e-3, 50)]):
+            want = sla.expm(1j * O.alg_matrix(3, th * d))
+            got = O.to_matrix(3, O.expi(3, 1.0, th * d))
+            np.testing.assert_allclose(got, want, atol=2e-14)
+
+
 @pytest.mark.parametrize("nc", [2, 3])
 def test_log_inverts_exp_and_matches_logm(nc):
     for scale in [1e-6, 0.05, 0.5, 1.5]:
