@@ -161,6 +161,13 @@ int tcym_level(tcym_ctx *ctx, int u, int e, int *level, int N[3]);
    so far (block_frequency 0: never).  *nblocked (may be NULL) receives b. */
 int tcym_cool_blocked(tcym_ctx *ctx, int u_hot, int u_cold, int nsteps, double dtau, int block_frequency,
                       int max_blocking, int *nblocked);
+/* Cooling::Perform with Calibrate = 0 (GRADIENTFLOW/Cooling.cpp:69-147) in place on link
+   slot u: flow steps step0+1 .. nsteps, blocked as in tcym_cool_blocked with the blocking
+   count continuing from *nblocked (in/out).  tcym_cool_blocked is a copy followed by this
+   call from step 0; a caller that checks the cooled links every so many steps (as
+   Cooling::Perform does every 100, :113-119) continues the cooling in pieces. */
+int tcym_cool_continue(tcym_ctx *ctx, int u, int step0, int nsteps, double dtau, int block_frequency,
+                       int max_blocking, int *nblocked);
 /* Cooling::ContinueCooling with GradientFlow::CalibrationUpdate (GRADIENTFLOW/Cooling.cpp:181-195,
    69-147; GradientFlow.cpp:168-188): u_work := u_src with E slot e_work zero, then the
    calibration flow steps step0+1 .. nsteps (the reference's CoolingSteps continue from

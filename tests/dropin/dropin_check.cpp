@@ -18,6 +18,9 @@
 
 #include "tcym_dropin.hpp"
 
+DOUBLE Qs = 1.0;
+DOUBLE n0 = 1.0;
+
 static void rd(const char* f, double* p, size_t n) {
     FILE* fp = fopen(f, "rb");
     if (!fp || fread(p, sizeof(double), n, fp) != n) { fprintf(stderr, "cannot read %s\n", f); exit(3); }
@@ -39,7 +42,7 @@ int main(int argc, char** argv) {
     rd(argv[3], U.U, nu);
     rd(argv[4], E.E, ne);
     try {
-        Dynamics::dTau() = 0.05;
+        Dynamics::dTau = 0.05;
         for (int k = 0; k < 4; k++) Dynamics::Update(&U, &E);
         const double ncs = ChernSimonsNumber::NCsDot(&U, &E);
         Observables::Bulk::Update(&U, &E);
@@ -47,7 +50,7 @@ int main(int argc, char** argv) {
         Observables::Unitarity::UpdateViolation(0, N - 1, 0, N - 1, 0, N - 1, &U);
         Observables::HardScales::Update(&U);
         printf("%.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", ncs, Observables::Bulk::T00(),
-               Observables::GaussLaw::GlobalMaxViolation(), Observables::Unitarity::GlobalMaxViolation(),
+               Observables::GaussLaw::GlobalMaxViolation, Observables::Unitarity::GlobalMaxViolation,
                Observables::Bulk::TXX(), Observables::Bulk::TZZ(), Observables::HardScales::LambdaXX(),
                Observables::HardScales::LambdaZZ());
         wr(std::string(argv[5]) + "_U.bin", U.U, nu);
@@ -60,25 +63,28 @@ int main(int argc, char** argv) {
         wr(std::string(argv[5]) + "_E2.bin", E.E, ne);
 
         // a cold copy blocked once after flow step 3 of 8
-        GradientFlow::SqrtDcTauOverSpacing() = 0.2;
-        Cooling::BlockFrequency() = 3;
+        GradientFlow::SqrtDcTauOverSpacing = 0.2;
+        Cooling::BlockFrequency = 3;
         GaugeLinks UC(N, N, N, 1.0, 1.0, 1.0);
         Cooling::CoolNSave("cold", 8, 1, &U, &UC);
         wr(std::string(argv[5]) + "_C.bin", UC.U, (size_t)3*tcym_dropin::LinkSize()*UC.Volume);
-        printf("%d %.17g %d %d %.17g\n", UC.N[0], UC.a[0], Cooling::BlockingNumber(), GradientFlow::CoolingSteps(),
+        printf("%d %.17g %d %d %.17g\n", UC.N[0], UC.a[0], Cooling::DynamicFields::BlockingNumber, GradientFlow::CoolingSteps,
                GradientFlow::CoolingTime());
         // CoolingMethod::Start with 4 flow steps, then Calibrate: 5 more calibration
         // steps, blocked after step 6, logged to <dir>/VacuumCoolingT0.4ID7.log
-        ChernSimonsNumber::CoolingMethod::StandardCoolingMaxSteps() = 4;
-        ChernSimonsNumber::CoolingMethod::CalibrationCoolingMaxSteps() = 5;
-        ChernSimonsNumber::CoolingMethod::CalibrationBlockingLevel() = 1;
-        ChernSimonsNumber::CoolingMethod::Start(&U);
-        Cooling::BlockFrequency() = 6;
+        ChernSimonsNumber::CoolingMethod::StandardCoolingMaxSteps = 4;
+        ChernSimonsNumber::CoolingMethod::CalibrationCoolingMaxSteps = 5;
+        ChernSimonsNumber::CoolingMethod::CalibrationBlockingLevel = 1;
+        GLinks::U = &U;
+        EFields::E = &E;
+        ChernSimonsNumber::CoolingMethod::Init();
+        ChernSimonsNumber::CoolingMethod::Start();
+        Cooling::BlockFrequency = 6;
         IO::SetOutputDirectory(argv[6]);
-        RandomNumberGenerator::MySEED() = 7;
-        ChernSimonsNumber::CoolingMethod::Calibrate(&U);
-        printf("%.17g %d %d %.17g\n", ChernSimonsNumber::DeltaNCsCalibration(), Cooling::BlockingNumber(),
-               GradientFlow::CoolingSteps(), GradientFlow::CoolingTime());
+        RandomNumberGenerator::MySEED = 7;
+        ChernSimonsNumber::CoolingMethod::Calibrate();
+        printf("%.17g %d %d %.17g\n", ChernSimonsNumber::DeltaNCsCalibration, Cooling::DynamicFields::BlockingNumber,
+               GradientFlow::CoolingSteps, GradientFlow::CoolingTime());
         // the hot fields as text (<dir>/UOutT0.4ID7.txt, EOutT0.4ID7.txt), read
         // back into new containers
         IO::SaveConfiguration("UOutT0.4", "EOutT0.4", &U, &E);
@@ -93,12 +99,12 @@ int main(int argc, char** argv) {
         GaugeLinks UT(N, N, N, 1.0, 1.0, 1.0);
         ElectricFields ET(N, N, N, 1.0, 1.0, 1.0);
         InitialConditions::SetZero(&UT, &ET);
-        ThermalDynamics::beta() = 4.0;
+        ThermalDynamics::beta = 4.0;
         ThermalDynamics::Thermalize(2, 1.6, &UT, &ET);
         wr(std::string(argv[5]) + "_TU.bin", UT.U, nu);
         wr(std::string(argv[5]) + "_TE.bin", ET.E, ne);
         // quasi-particles (Qs 1.5, n0 1, key MySEED = 7), Gauss law restored
-        InitialConditions::Qs() = 1.5;
+        Qs = 1.5;
         InitialConditions::SetQuasiParticles(&UT, &ET);
         wr(std::string(argv[5]) + "_QU.bin", UT.U, nu);
         wr(std::string(argv[5]) + "_QE.bin", ET.E, ne);

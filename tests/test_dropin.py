@@ -114,3 +114,141 @@ def test_dropin_matches_oracle(nc, tmp_path):
     qp.gauss_restore()
     np.testing.assert_allclose(np.fromfile(tmp_path / "out_QU.bin"), qp.U, atol=1e-12)
     np.testing.assert_allclose(np.fromfile(tmp_path / "out_QE.bin"), qp.E, atol=1e-12)
+
+
+DRIVER = os.path.join(ROOT, "tests", "dropin", "_build", "sphaleron_driver")
+
+
+def test_sphaleron_driver_is_built():
+    assert os.path.exists(DRIVER), "run __graft_entry__.build() first"
+
+
+def oracle_sphaleron_run(nc, N, U0, E0):
+    """The sphaleron driver's run on the oracle: 40 leapfrog steps of 0.025,
+    cooled copies of 12 flow steps (dtau 1/8) every 5 steps with the geodesic
+    Delta N_CS, calibration (flow steps 13..18, blocked after 16) at the start
+    and every 20 steps, energy rows every 20 steps; then 3 Langevin steps and a
+    noise-field step."""
+    dt, dtau = 0.025, 0.125
+    lat = O.Lattice(nc, N)
+    lat.U[:], lat.E[:] = U0, E0
+    out = {"energy": [], "cool": [], "calib": [], "logs": {}}
+
+    def energy(t):
+        b, H = lat.bulk(), lat.hard_scales()
+        el, mg = 0.5 * b[:3].sum(), 0.5 * b[3:].sum()
+        tii = [0.5 * (b.sum() - 2 * (b[m] + b[3 + m])) for m in range(3)]
+        lam = [H.sum() - 2 * H[m] for m in range(3)]
+        out["energy"].append([t, el + mg] + tii + [el, mg] + lam)
+
+    def cool(U):
+        c = lat.copy()
+        c.U[:], c.E[:] = U, 0.0
+        c.flow(12, dtau)
+        return c.U.copy()
+
+    state = {"calib": 0.0, "rt": 0.0}
+
+    def calibrate(t, Unew):
+        w = lat.copy()
+        w.U[:], w.E[:] = Unew, 0.0
+        acc, ctau, nb, rows = 0.0, 12 * dtau, 0, []
+
+        def row(w):
+            b = w.bulk()
+            cb = (w.p.a[0] / w.p.a[0] ** 3) ** 2
+            return [ctau, 0.5 * cb * b[3:].sum(), 0.5 * cb * b[:3].sum(), acc, w.vacuum_ncs()]
+        rows.append(row(w))
+        for k in range(13, 19):
+            acc += w.flow_calibration(1, dtau)
+            ctau += dtau * (w.p.a[0] ** 3) ** (2.0 / 3.0)
+            rows.append(row(w))
+            if k == 16 and nb < 1:
+                w = w.blocked()
+                nb += 1
+        out["logs"]["%g" % t] = np.array(rows)
+        out["calib"].append([t, acc, state["calib"], state["rt"], state.get("rtprev", 0.0)])
+        state["rtprev"] = state["rt"]
+        state["calib"] = acc
+
+    energy(0.0)
+    Uold = cool(lat.U)
+    out["cool"].append([0.0, 0.0, 0.0])
+    calibrate(0.0, Uold)
+    for step in range(1, 41):
+        lat.leapfrog(1, dt)
+        t = step * dt
+        if step % 20 == 0:
+            energy(t)
+        if step % 5 == 0:
+            Unew = cool(lat.U)
+            if step == 5:
+                out["hot_map"] = lat.edotb_map()
+                Um, Em = lat.geodesic(Uold, Unew, 0.5)
+                out["cool_map"] = _with(lat, Um, Em).edotb_map()
+            state["rt"] += lat.delta_ncs_cooling(Uold, Unew)
+            out["cool"].append([t, state["rt"], 0.0])
+            Uold = Unew
+        if step % 20 == 0:
+            calibrate(t, Uold)
+    lat.langevin(3, 0.01, 1.0, 0.5, 7, 0)
+    Xi = lat.noise(7, 3, 1.0)
+    lat.langevin_xi(Xi, 0.01, 1.0)
+    out["U"], out["E"], out["box"] = lat.U.copy(), lat.E.copy(), lat.edotb_map()
+    return out
+
+
+def _with(lat, U, E):
+    c = lat.copy()
+    c.U[:], c.E[:] = U, E
+    return c
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nc", [2, 3])
+@pytest.mark.parametrize("resident", [0, 1])
+def test_sphaleron_driver_matches_oracle(nc, resident, tmp_path):
+    N = 8
+    lat = O.Lattice(nc, N)
+    lat.random(2025, 0.4, 0.3, restore=True)
+    lat.U.tofile(tmp_path / "U.bin")
+    lat.E.tofile(tmp_path / "E.bin")
+    print("sphaleron driver SU(%d) %d^3 resident %d" % (nc, N, resident), flush=True)
+    r = subprocess.run([DRIVER, str(nc), str(N), str(tmp_path / "U.bin"), str(tmp_path / "E.bin"), str(tmp_path),
+                        str(resident)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0].split() == ["refused", "3", "tSteps", "42", "tauLSteps", "4"], r.stdout
+    want = oracle_sphaleron_run(nc, N, lat.U, lat.E)
+    # CoolNSave (6 flow steps) + NCsDot of the cold copy with the hot E, twice,
+    # a leapfrog step between: the second value must come from the second copy
+    hot = _with(lat, want["U"], want["E"])
+    for k, got in enumerate(map(float, lines[1].split())):
+        cold = _with(hot, hot.U, hot.E)
+        cold.flow(6, 0.125)
+        assert abs(got - cold.ncsdot(cold.U, hot.E)) < 1e-12, (k, got)
+        hot.leapfrog(1, 0.025)
+    np.testing.assert_allclose(np.fromfile(tmp_path / "final_U.bin"), want["U"], atol=1e-11)
+    np.testing.assert_allclose(np.fromfile(tmp_path / "final_E.bin"), want["E"], atol=1e-11)
+    scale = abs(want["energy"][0][1])
+    np.testing.assert_allclose(np.loadtxt(tmp_path / "EnergyID7.txt"), np.array(want["energy"]), rtol=1e-10,
+                               atol=1e-12 * scale)
+    np.testing.assert_allclose(np.loadtxt(tmp_path / "CoolNCsID7.txt"), np.array(want["cool"]), rtol=1e-9, atol=1e-13)
+    np.testing.assert_allclose(np.loadtxt(tmp_path / "CalibNCsID7.txt"), np.array(want["calib"]), rtol=1e-9,
+                               atol=1e-13)
+    # the calibration logs, the maps and the box map in the stream's default 6 digits
+    for t, rows in want["logs"].items():
+        np.testing.assert_allclose(np.loadtxt(tmp_path / ("VacuumCoolingT%sID7.log" % t)), rows, rtol=1e-5, atol=1e-9)
+    for name, m in (("HotEdotBMapC0", want["hot_map"]), ("CoolEdotBMapC0", want["cool_map"])):
+        got = np.loadtxt(tmp_path / (name + "ID7.txt"))
+        assert got.shape == (N ** 3, 4)
+        xyz = got[:, :3].astype(int)
+        assert [tuple(v) for v in xyz[:3]] == [(0, 0, 0), (1, 0, 0), (2, 0, 0)]
+        np.testing.assert_allclose(got[:, 3], m[xyz[:, 0] + N * (xyz[:, 1] + N * xyz[:, 2])], rtol=1e-5,
+                                   atol=1e-6 * np.abs(m).max())
+    box = np.loadtxt(tmp_path / "BoxMapID7.txt")
+    assert [tuple(v) for v in box[:, :3].astype(int)] == [(1, 0, 3), (2, 0, 3)]
+    np.testing.assert_allclose(box[:, 3], want["box"][[1 + N * N * 3, 2 + N * N * 3]], rtol=1e-5, atol=1e-9)
+    # the precision checks of the loop
+    assert "#GAUSS LAW VIOLATION AT T=0.625 MAX=" in r.stderr and "#UNITARITY VIOLATION AT T=0.625 MAX=" in r.stderr
+    assert r.stderr.count("#CALIBRATION COMPLETED") == 3

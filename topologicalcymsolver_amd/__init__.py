@@ -31,7 +31,7 @@ EXPORTS = [
     "tcym_noise", "tcym_flow", "tcym_cool", "tcym_cooling_start", "tcym_cooling_update", "tcym_reunitarize",
     "tcym_edotb", "tcym_ncsdot", "tcym_ncsdot_noalloc", "tcym_edotb_map", "tcym_bulk", "tcym_hard_scales", "tcym_gauss", "tcym_unitarity",
     "tcym_gauss_restore", "tcym_geodesic", "tcym_delta_ncs_cooling", "tcym_vacuum_ncs",
-    "tcym_block", "tcym_level", "tcym_cool_blocked", "tcym_continue_cooling", "tcym_thermalize",
+    "tcym_block", "tcym_level", "tcym_cool_blocked", "tcym_cool_continue", "tcym_continue_cooling", "tcym_thermalize",
     "tcym_quasi_particles",
     "tcym_timing", "tcym_kernel_time", "tcym_timing_reset", "tcym_sync", "tcym_device_bytes",
     "tcym_get_params",
@@ -118,6 +118,7 @@ def load_library(path=LIB_PATH):
         "tcym_vacuum_ncs": [V, I, _dp], "tcym_timing": [V, I],
         "tcym_block": [V, I, I], "tcym_level": [V, I, I, ctypes.POINTER(I), ctypes.POINTER(I)],
         "tcym_cool_blocked": [V, I, I, I, D, I, I, ctypes.POINTER(I)],
+        "tcym_cool_continue": [V, I, I, I, D, I, I, ctypes.POINTER(I)],
         "tcym_continue_cooling": [V, I, I, I, I, I, D, D, I, I, ctypes.POINTER(I), _dp, _dp],
         "tcym_kernel_time": [V, ctypes.c_char_p, _dp, LL], "tcym_timing_reset": [V], "tcym_sync": [V],
         "tcym_device_bytes": [V, LL],

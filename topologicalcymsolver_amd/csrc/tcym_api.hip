@@ -732,26 +732,34 @@ static int perform_cooling(tcym_ctx* c, int u, int e, int step0, int nsteps, dou
     };
     double* row = log;
     if (calibrate && row) { if ((r = log_row(row))) return r; row += 5; }
-    for (int k = step0 + 1; k <= nsteps; k++) {
+    // the step after which the next blocking is due (BlockFrequency 8^BlockingNumber,
+    // Cooling.cpp:123), or 0 when no more blocking happens
+    auto next_block = [&](int k) -> long {
+        if (bfreq <= 0 || *nblock >= maxb) return 0;
+        long every = bfreq;
+        for (int l = 0; l < *nblock; l++) every *= 8;
+        return ((long)k/every + 1)*every;
+    };
+    for (int k = step0; k < nsteps;) {
         Geom q;
         if ((r = c->at(q, u, calibrate ? e : -1))) return r;
         const double D = tcym_ctx::flow_dctau(q, dtau);
+        const long nb = next_block(k);
+        // standard cooling runs the steps up to the next blocking in one call
+        const int k1 = calibrate ? k + 1 : (int)(nb > 0 && nb < nsteps ? nb : nsteps);
         if (calibrate) {
             double d = 0.0;
             if ((r = tcym_flow(c, u, e, 1, dtau, 1, &d))) return r;
             acc += d;
-        } else if ((r = run_update(c, 3, u, -1, -1, 1, dtau, 0.0, 0.0, 0, 0, false))) {
+        } else if ((r = run_update(c, 3, u, -1, -1, k1 - k, dtau, 0.0, 0.0, 0, 0, false))) {
             return r;
         }
-        ctau += D;
+        for (int j = k; j < k1; j++) ctau += D;
+        k = k1;
         if (calibrate && row) { if ((r = log_row(row))) return r; row += 5; }
-        if (bfreq > 0 && *nblock < maxb) {
-            long every = bfreq;
-            for (int l = 0; l < *nblock; l++) every *= 8;    // BlockFrequency 8^BlockingNumber (Cooling.cpp:123)
-            if (k % every == 0) {
-                if ((r = tcym_block(c, u, calibrate ? e : -1))) return r;
-                (*nblock)++;
-            }
+        if (nb > 0 && k == nb) {
+            if ((r = tcym_block(c, u, calibrate ? e : -1))) return r;
+            (*nblock)++;
         }
     }
     if (dncs) *dncs = acc;
@@ -767,6 +775,15 @@ int tcym_cool_blocked(tcym_ctx* c, int u_hot, int u_cold, int nsteps, double dta
     if ((r = perform_cooling(c, u_cold, -1, 0, nsteps, dtau, block_frequency, max_blocking, &nb, 0, 0.0, nullptr, nullptr))) return r;
     if (nblocked) *nblocked = nb;
     return TCYM_OK;
+}
+int tcym_cool_continue(tcym_ctx* c, int u, int step0, int nsteps, double dtau, int block_frequency, int max_blocking,
+                       int* nblocked) {
+    if (!c || !nblocked) return TCYM_E_ARG;
+    if (step0 < 0 || nsteps < step0 || block_frequency < 0 || max_blocking < 0 || *nblocked < 0)
+        return c->fail(TCYM_E_ARG, "bad step or blocking counts");
+    int r = c->useU(u);
+    if (r) return r;
+    return perform_cooling(c, u, -1, step0, nsteps, dtau, block_frequency, max_blocking, nblocked, 0, 0.0, nullptr, nullptr);
 }
 int tcym_continue_cooling(tcym_ctx* c, int u_src, int u_work, int e_work, int step0, int nsteps, double dtau,
                           double ctau0, int block_frequency, int max_blocking, int* nblocked, double* dncs, double* log) {
