@@ -48,8 +48,8 @@ def _compile(src, build_dir=None, extra=()):
 
 
 def build(verbose=False, jobs=None, out=None, defines=()):
-    """Compile and link; `out`/`defines` build a variant library elsewhere
-    (tuning knobs such as -DTCYM_PIN_EVERY=2)."""
+    """Compile and link; `out`/`defines` build a library elsewhere with extra
+    compiler options."""
     lib = out or LIB
     bdir = os.path.join(os.path.dirname(lib), "_build") if out else BUILD
     os.makedirs(bdir, exist_ok=True)

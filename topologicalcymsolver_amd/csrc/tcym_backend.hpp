@@ -132,10 +132,7 @@ inline long padded_sites(long V) { return (V + 63) & ~63L; }
 // go to the wider patch, for coalescing).  Linear walk (pxs == 0): a block
 // covers kBlock consecutive sites in natural order; it is used when it pads
 // less than every patch shape (small or odd lattices).
-#ifndef TCYM_COL_ROWS
-#define TCYM_COL_ROWS 16
-#endif
-constexpr int kColRows = TCYM_COL_ROWS;
+constexpr int kColRows = 16;
 struct WalkShape {
     int pxs;           // log2 of the patch width in x, 0 for the linear walk
     int tcx, tcy;      // columns in x and y (patch walk)

@@ -5,22 +5,16 @@
 //   SU(3): 9 complex, column-major (i,j) -> i+3j, stored re,im (SUNc/SU3/GroupOperations.cpp:4-14)
 //   algebra: t^a = sigma_a/2 or lambda_a/2 (Gell-Mann order)
 // Every routine is templated on the arithmetic type T (double for the FP64
-// form, float for the FP32 form).  Reunitarisation runs in Reunit<T>: double,
-// or float for the FP32 form when TCYM_REUNIT_F32 is 1.
+// form, float for the FP32 form).  Reunitarisation runs in double (Reunit<T>)
+// in both forms.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <type_traits>
 
-#ifndef TCYM_REUNIT_F32
-#define TCYM_REUNIT_F32 0
-#endif
-
 namespace tcym {
 
-template <typename T> struct ReunitOf { using type = double; };
-template <> struct ReunitOf<float> { using type = typename std::conditional<TCYM_REUNIT_F32 != 0, float, double>::type; };
-template <typename T> using Reunit = typename ReunitOf<T>::type;
+template <typename T> using Reunit = double;
 // 1/sqrt(x): correctly rounded division and square root in double; the
 // hardware reciprocal square root refined by one Newton step in float
 __device__ __forceinline__ double rsqrt_of(double x) { return 1.0/__builtin_sqrt(x); }
@@ -154,6 +148,10 @@ template <typename T> struct SU2 {
         R n = rsqrt_reunit<T>(a*a + b*b + c*c + d*d);
         U.v[0] = T(a*n); U.v[1] = T(b*n); U.v[2] = T(c*n); U.v[3] = T(d*n);
     }
+    // the SU(3) two-column forms reduce to the whole quaternion
+    __device__ static __forceinline__ Mat mul01(const Mat& A, const Mat& B) { return mul(A, B); }
+    __device__ static __forceinline__ void reunitarize2(Mat& U) { reunitarize(U); }
+    __device__ static __forceinline__ void complete(Mat&) {}
     __device__ static __forceinline__ T fstruct(int a, int b, int c) { return T(((a-b)*(a-c)*(c-b))/2); }
 };
 
@@ -446,41 +444,73 @@ template <typename T> struct SU3 {
         }
         return dsqrt(n);
     }
-    // Gram-Schmidt on rows 0,1; row 2 = conj(row0 x row1), in Reunit<T> arithmetic
-    __device__ static __forceinline__ void reunitarize(Mat& U) {
-        using R = Reunit<T>;
-        R r0r[3], r0i[3], r1r[3], r1i[3];
+    // columns 0 and 1 of op(A) B (column 2 of the result is left zero): the
+    // part of a link update that the reunitarisation and a two-column store read
+    __device__ static __forceinline__ Mat mul01(const Mat& A, const Mat& B) {
+        Mat C;
 #pragma unroll
-        for (int j = 0; j < 3; j++) { r0r[j] = U.r[3*j]; r0i[j] = U.i[3*j]; r1r[j] = U.r[1 + 3*j]; r1i[j] = U.i[1 + 3*j]; }
+        for (int row = 0; row < 3; row++) {
+#pragma unroll
+            for (int col = 0; col < 2; col++) {
+                T sr = T(0), si = T(0);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int ia = row + 3*k, ib = k + 3*col;
+                    sr = fmat(A.r[ia], B.r[ib], sr); sr = fmat(-A.i[ia], B.i[ib], sr);
+                    si = fmat(A.r[ia], B.i[ib], si); si = fmat(A.i[ia], B.r[ib], si);
+                }
+                C.r[row + 3*col] = sr; C.i[row + 3*col] = si;
+            }
+            C.r[row + 6] = T(0); C.i[row + 6] = T(0);
+        }
+        return C;
+    }
+    // Gram-Schmidt on columns 0 and 1 in Reunit<T> arithmetic; with full,
+    // column 2 = conj(col0 x col1) in the same arithmetic, else untouched
+    template <bool full>
+    __device__ static __forceinline__ void reunitarize_cols(Mat& U) {
+        using R = Reunit<T>;
+        R c0r[3], c0i[3], c1r[3], c1i[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) { c0r[j] = U.r[j]; c0i[j] = U.i[j]; c1r[j] = U.r[3 + j]; c1i[j] = U.i[3 + j]; }
         R n = 0;
 #pragma unroll
-        for (int j = 0; j < 3; j++) n += r0r[j]*r0r[j] + r0i[j]*r0i[j];
+        for (int j = 0; j < 3; j++) n += c0r[j]*c0r[j] + c0i[j]*c0i[j];
         n = rsqrt_reunit<T>(n);
 #pragma unroll
-        for (int j = 0; j < 3; j++) { r0r[j] *= n; r0i[j] *= n; }
-        R pr = 0, pi = 0;   // <r0, r1>
+        for (int j = 0; j < 3; j++) { c0r[j] *= n; c0i[j] *= n; }
+        R pr = 0, pi = 0;   // <c0, c1>
 #pragma unroll
-        for (int j = 0; j < 3; j++) { pr += r0r[j]*r1r[j] + r0i[j]*r1i[j]; pi += r0r[j]*r1i[j] - r0i[j]*r1r[j]; }
+        for (int j = 0; j < 3; j++) { pr += c0r[j]*c1r[j] + c0i[j]*c1i[j]; pi += c0r[j]*c1i[j] - c0i[j]*c1r[j]; }
         n = 0;
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            r1r[j] -= pr*r0r[j] - pi*r0i[j];
-            r1i[j] -= pr*r0i[j] + pi*r0r[j];
-            n += r1r[j]*r1r[j] + r1i[j]*r1i[j];
+            c1r[j] -= pr*c0r[j] - pi*c0i[j];
+            c1i[j] -= pr*c0i[j] + pi*c0r[j];
+            n += c1r[j]*c1r[j] + c1i[j]*c1i[j];
         }
         n = rsqrt_reunit<T>(n);
 #pragma unroll
-        for (int j = 0; j < 3; j++) { r1r[j] *= n; r1i[j] *= n; }
-#pragma unroll
         for (int j = 0; j < 3; j++) {
-            const int a = (j + 1) % 3, b = (j + 2) % 3;
-            R cr = r0r[a]*r1r[b] - r0i[a]*r1i[b] - (r0r[b]*r1r[a] - r0i[b]*r1i[a]);
-            R ci = r0r[a]*r1i[b] + r0i[a]*r1r[b] - (r0r[b]*r1i[a] + r0i[b]*r1r[a]);
-            U.r[3*j] = T(r0r[j]); U.i[3*j] = T(r0i[j]);
-            U.r[1 + 3*j] = T(r1r[j]); U.i[1 + 3*j] = T(r1i[j]);
-            U.r[2 + 3*j] = T(cr); U.i[2 + 3*j] = T(-ci);
+            c1r[j] *= n; c1i[j] *= n;
+            U.r[j] = T(c0r[j]); U.i[j] = T(c0i[j]);
+            U.r[3 + j] = T(c1r[j]); U.i[3 + j] = T(c1i[j]);
+        }
+        if constexpr (full) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const int a = (i + 1) % 3, b = (i + 2) % 3;
+                const R cr = c0r[a]*c1r[b] - c0i[a]*c1i[b] - (c0r[b]*c1r[a] - c0i[b]*c1i[a]);
+                const R ci = c0r[a]*c1i[b] + c0i[a]*c1r[b] - (c0r[b]*c1i[a] + c0i[b]*c1r[a]);
+                U.r[6 + i] = T(cr); U.i[6 + i] = T(-ci);
+            }
         }
     }
+    // the nearest-by-Gram-Schmidt special unitary matrix: columns 0, 1
+    // orthonormalised, column 2 = conj(col0 x col1)
+    __device__ static __forceinline__ void reunitarize(Mat& U) { reunitarize_cols<true>(U); }
+    // columns 0 and 1 only (a link stored as two columns, or completed later)
+    __device__ static __forceinline__ void reunitarize2(Mat& U) { reunitarize_cols<false>(U); }
     __device__ static T fstruct(int a, int b, int c);
 };
 
