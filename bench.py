@@ -160,9 +160,10 @@ def run_batched(T, count, seed, precision, lang_steps, seg, cool_every, cool_ste
     t0 = time.perf_counter()
     iters = b.initial_condition(seed, 0.2, 0.3, maxiter=10000, tol=1e-14)
     t_ic = time.perf_counter() - t0
-    choice, best_us = (None, None)
+    choice, best_us, fchoice, fbest_us = None, None, None, None
     if tune:
         choice, best_us = b.tune(mode=1, dt=dt, reps=3)
+        fchoice, fbest_us = b.tune(mode=3, dt=dtau, reps=3)
     # warm-up: one segment on the same path (the graph is captured here), then restart the fields
     links = [b.get_links(e) for e in range(count)]
     efield = [b.get_efield(e) for e in range(count)]
@@ -198,7 +199,8 @@ def run_batched(T, count, seed, precision, lang_steps, seg, cool_every, cool_ste
     ga = b.gauss()
     b.close()
     links_total = sum(3 * L ** 3 for L in sizes)
-    return dict(sizes=sizes, wall=wall, t_ic=t_ic, iters=iters, choice=choice, best_us=best_us,
+    return dict(sizes=sizes, wall=wall, t_ic=t_ic, iters=iters, choice=choice, best_us=best_us, fchoice=fchoice,
+                fbest_us=fbest_us,
                 lu=links_total * (lang_steps + n_flow) / wall, links=links_total, n_lang=lang_steps, n_flow=n_flow,
                 ncsdot=np.concatenate(ncsdot, axis=1), dncs=np.cumsum(np.array(dncs).T, axis=1), kernels=kern,
                 unitarity=float(un[:, 0].max()), gauss=float(np.abs(ga[:, 0]).max()))
@@ -376,7 +378,8 @@ def main():
         bkw = dict(lang_steps=args.batch_steps, seg=10, cool_every=100, cool_steps=200, dt=0.05, dtau=0.02,
                    damping=0.1, temp=0.5)
         b32 = run_batched(T, args.batch_count, 4004, T.FP32, tune=True, **bkw)
-        log("# batched FP32: %.3f s, %.3e link-updates/s, tuner %s" % (b32["wall"], b32["lu"], b32["choice"]))
+        log("# batched FP32: %.3f s, %.3e link-updates/s, tuner %s / flow %s" % (b32["wall"], b32["lu"], b32["choice"],
+                                                                                   b32["fchoice"]))
         b64 = run_batched(T, args.batch_count, 4004, T.FP64, tune=True, **bkw)
         log("# batched FP64: %.3f s, %.3e link-updates/s" % (b64["wall"], b64["lu"]))
         kd = max(b32["kernels"], key=lambda k: b32["kernels"][k]["avg_us"] * b32["kernels"][k]["launches"])
@@ -397,8 +400,10 @@ def main():
                         % (args.batch_count, {L: b32["sizes"].count(L) for L in (16, 24, 32, 40, 48)}, args.batch_steps),
             "link_updates_per_s": b32["lu"], "wall_s": b32["wall"], "links": b32["links"],
             "langevin_steps": b32["n_lang"], "flow_steps": b32["n_flow"],
-            "tuner_choice": {"shape": b32["choice"][0], "blocks_per_cu": b32["choice"][1], "xcd_split": b32["choice"][2],
-                             "best_langevin_us": b32["best_us"]},
+            "tuner_choice": {"langevin": {"shape": b32["choice"][0], "blocks_per_cu": b32["choice"][1],
+                                          "xcd_split": b32["choice"][2], "best_us": b32["best_us"]},
+                             "flow": {"shape": b32["fchoice"][0], "blocks_per_cu": b32["fchoice"][1],
+                                      "xcd_split": b32["fchoice"][2], "best_us": b32["fbest_us"]}},
             "kernels": b32["kernels"], "roofline": roof,
             "fp64_link_updates_per_s": b64["lu"], "fp32_vs_fp64": b32["lu"] / b64["lu"],
             "max_abs_delta_ncs_diff_vs_fp64": float(np.abs(b32["dncs"] - b64["dncs"]).max()),

@@ -97,8 +97,12 @@ int tcym_batch_unitarity(tcym_batch *b, int u, double *out);
 
 /* Autotuner of the update kernels: times every candidate walk shape
  * (canonical, linear, 64/32/16-wide patches) and persistent grid size on
- * scratch copies of the fields with `mode` (0 leapfrog, 1 Langevin, 3 flow)
- * and keeps the fastest.  Results do not depend on the choice.  choice
+ * scratch copies of the fields and keeps the fastest for one pass: mode 0 or
+ * 1 tunes the leapfrog and Langevin updates (repeated launches on one
+ * input), mode 3 the flow (sequences of flow steps alternating between two
+ * link buffers with two-column stores, as in a cooling episode); each pass
+ * keeps its own choice.  The timing runs overwrite the free slots
+ * TCYM_U_AUX and TCYM_E_AUX.  Results do not depend on the choice.  choice
  * receives {shape, blocks per CU, XCD split} and us the best time per
  * update step in microseconds (both may be NULL). */
 int tcym_batch_tune(tcym_batch *b, int mode, double dt, int reps, int choice[3], double *us);

@@ -168,6 +168,17 @@ def test_autotuned_batch_is_bit_identical(nc, sizes):
     for e, s in enumerate(singles):
         s.langevin_update(2, 0.05, 0.1, 0.5, NOISE0 + e, 3)
         assert traj[e, 0] == s.ncs_dot()
+    for e, s in enumerate(singles):                   # the second segment of evolve
+        s.langevin_update(2, 0.05, 0.1, 0.5, NOISE0 + e, 5)
+    assert_same_fields(b, singles)
+    # the flow pass keeps a choice of its own
+    fchoice, fus = b.tune(mode=3, dt=0.02, reps=2)
+    assert fchoice[0] in (-1, 0, 4, 5, 6) and fchoice[1] in (2, 4, 8, 16) and fus > 0
+    assert_same_fields(b, singles)
+    b.flow(5, 0.02, u=T.U_HOT)
+    for s in singles:
+        s.gradient_flow(5, 0.02, u=T.U_HOT, e=-1)
+    assert_same_fields(b, singles)
 
 
 def test_batch_rejects_bad_arguments():

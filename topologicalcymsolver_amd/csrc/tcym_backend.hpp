@@ -26,28 +26,30 @@ struct KGeom {
 };
 
 // One lattice of a batch (device table).  Each pass over a batch has its own
-// tiling: `gu`/`utile0` for the update kernels (the autotuner picks the walk
-// shape), `gr`/`ltile0`/`stile0` for the measurement kernels (the shape a
-// single lattice of the same size uses, so that their sums are bit-identical
-// to the single-lattice ones).
+// tiling: `gu`/`utile0` for the leapfrog and Langevin updates and `gf`/`ftile0`
+// for the flow (the autotuner picks each walk shape), `gr`/`ltile0`/`stile0`
+// for the measurement kernels (the shape a single lattice of the same size
+// uses, so that their sums are bit-identical to the single-lattice ones).
 struct BatchEns {
-    KGeom gu, gr;
+    KGeom gu, gf, gr;
     uint64_t u_off, e_off;     // element offsets of the lattice's link / E records
     uint64_t seed;             // Langevin noise key
-    uint32_t utile0;           // first tile of its update pass (3 per site block)
+    uint32_t utile0;           // first tile of its leapfrog / Langevin pass (3 per site block)
+    uint32_t ftile0;           // first tile of its flow pass
     uint32_t ltile0;           // first tile of its measurement pass over links
     uint32_t stile0;           // first tile of its measurement pass over sites
-    uint32_t pad;
 };
 
 // device tables of a batch handed to the kernels (owned by the batch object)
 struct BatchDev {
     Geom g;                            // spacings (common to the batch); extents unused
     const BatchEns* ens;
-    const uint32_t* utiles; uint32_t n_utiles;   // tile -> lattice, update pass
+    const uint32_t* utiles; uint32_t n_utiles;   // tile -> lattice, leapfrog / Langevin pass
+    const uint32_t* ftiles; uint32_t n_ftiles;   // tile -> lattice, flow pass
     const uint32_t* ltiles; uint32_t n_ltiles;   // tile -> lattice, measurement pass over links
     const uint32_t* stiles; uint32_t n_stiles;   // tile -> lattice, measurement pass over sites
-    int grid_update;                   // persistent grid of the update kernels (blocks)
+    int grid_update;                   // persistent grid of the leapfrog / Langevin kernels (blocks)
+    int grid_flow;                     // persistent grid of the flow kernel
     int grid_measure;                  // persistent grid of the measurement kernels
     const uint32_t* step;              // device step counter (Philox step of step offset 0)
 };

@@ -75,9 +75,10 @@ struct tcym_batch : tcym::DeviceQueue {
     std::vector<BatchEns> ens;
     BatchEns* ens_dev = nullptr;
     uint32_t* utiles = nullptr;
+    uint32_t* ftiles = nullptr;
     uint32_t* ltiles = nullptr;
     uint32_t* stiles = nullptr;
-    uint32_t n_utiles = 0, n_ltiles = 0, n_stiles = 0;
+    uint32_t n_utiles = 0, n_ftiles = 0, n_ltiles = 0, n_stiles = 0;
     uint32_t* red_l0 = nullptr;     // per lattice: first tile and tile count of the link / site measurement passes
     uint32_t* red_ln = nullptr;
     uint32_t* red_s0 = nullptr;
@@ -91,9 +92,11 @@ struct tcym_batch : tcym::DeviceQueue {
     double* traj = nullptr;
     long traj_cap = 0;
     int cus = 256;
-    int shape = -1;                 // walk shape of the update kernels (see update_shape)
-    int per_cu = 8;                 // persistent blocks per CU of the update kernels
-    bool xcd_split = true;
+    // per pass (0: leapfrog and Langevin, 1: flow): walk shape of the update
+    // kernel (see update_shape), persistent blocks per CU, XCD ranges
+    int shape[2] = {-1, -1};
+    int per_cu[2] = {8, 8};
+    bool xcd_split[2] = {true, true};
     uint64_t seed_set = ~0ull;      // key base of the Langevin noise in the table
     struct Graph { hipGraph_t graph = nullptr; hipGraphExec_t exec = nullptr; };
     std::map<std::string, Graph> graphs;
@@ -127,9 +130,11 @@ struct tcym_batch : tcym::DeviceQueue {
         d.g = g;
         d.ens = ens_dev;
         d.utiles = utiles; d.n_utiles = n_utiles;
+        d.ftiles = ftiles; d.n_ftiles = n_ftiles;
         d.ltiles = ltiles; d.n_ltiles = n_ltiles;
         d.stiles = stiles; d.n_stiles = n_stiles;
-        d.grid_update = grid_for(n_utiles, per_cu, xcd_split);
+        d.grid_update = grid_for(n_utiles, per_cu[0], xcd_split[0]);
+        d.grid_flow = grid_for(n_ftiles, per_cu[1], xcd_split[1]);
         d.grid_measure = grid_for(std::max(n_ltiles, n_stiles), 8, true);
         d.step = counters;
         return d;
@@ -141,31 +146,35 @@ struct tcym_batch : tcym::DeviceQueue {
         }
         graphs.clear();
     }
-    // tile tables for the update walk shape `shape_` (measurement passes always
-    // use the canonical shape of each lattice)
-    int build_tables(int shape_) {
-        std::vector<uint32_t> ut, lt, st, l0, ln, s0, sn;
+    // tile tables for the walk shapes su (leapfrog, Langevin) and sf (flow)
+    // (measurement passes always use the canonical shape of each lattice)
+    int build_tables(int su, int sf) {
+        std::vector<uint32_t> ut, ft, lt, st, l0, ln, s0, sn;
         ens.assign(count, BatchEns{});
         for (int e = 0; e < count; e++) {
             const Geom& q = geo[e];
-            const WalkShape wu = update_shape(q, shape_), wr = tcym::walk_shape(q);
+            const WalkShape wu = update_shape(q, su), wf = update_shape(q, sf), wr = tcym::walk_shape(q);
             BatchEns& b = ens[e];
             b.gu = kgeom(q, wu);
+            b.gf = kgeom(q, wf);
             b.gr = kgeom(q, wr);
             b.u_off = (uint64_t)u_off[e];
             b.e_off = (uint64_t)e_off[e];
             b.seed = seed_set == ~0ull ? 0 : seed_set + (uint64_t)e;
             b.utile0 = (uint32_t)ut.size();
+            b.ftile0 = (uint32_t)ft.size();
             b.ltile0 = (uint32_t)lt.size();
             b.stile0 = (uint32_t)st.size();
             l0.push_back(b.ltile0); ln.push_back((uint32_t)(3*wr.blocks));
             s0.push_back(b.stile0); sn.push_back((uint32_t)wr.blocks);
             ut.insert(ut.end(), (size_t)(3*wu.blocks), (uint32_t)e);
+            ft.insert(ft.end(), (size_t)(3*wf.blocks), (uint32_t)e);
             lt.insert(lt.end(), (size_t)(3*wr.blocks), (uint32_t)e);
             st.insert(st.end(), (size_t)wr.blocks, (uint32_t)e);
         }
-        if (ut.size() >= 0xffffffffull || lt.size() >= 0xffffffffull) return fail(TCYM_E_UNSUPPORTED, "batch too large");
-        // the update table is sized for the largest candidate shape
+        if (ut.size() >= 0xffffffffull || ft.size() >= 0xffffffffull || lt.size() >= 0xffffffffull)
+            return fail(TCYM_E_UNSUPPORTED, "batch too large");
+        // the update tables are sized for the largest candidate shape
         size_t ucap = 0;
         for (const Geom& q : geo) {
             long m = 0;
@@ -174,6 +183,7 @@ struct tcym_batch : tcym::DeviceQueue {
         }
         int r;
         if ((r = alloc((void**)&utiles, sizeof(uint32_t)*ucap, "hipMalloc(tiles)"))) return r;
+        if ((r = alloc((void**)&ftiles, sizeof(uint32_t)*ucap, "hipMalloc(tiles)"))) return r;
         if ((r = alloc((void**)&ltiles, sizeof(uint32_t)*lt.size(), "hipMalloc(tiles)"))) return r;
         if ((r = alloc((void**)&stiles, sizeof(uint32_t)*st.size(), "hipMalloc(tiles)"))) return r;
         if ((r = alloc((void**)&ens_dev, sizeof(BatchEns)*count, "hipMalloc(lattice table)"))) return r;
@@ -181,9 +191,11 @@ struct tcym_batch : tcym::DeviceQueue {
             if ((r = alloc((void**)p, sizeof(uint32_t)*count, "hipMalloc(lattice table)"))) return r;
         const size_t np = std::max((size_t)lt.size(), (size_t)st.size()*6);
         if ((r = alloc((void**)&part, sizeof(double)*np, "hipMalloc(partials)"))) return r;
-        n_utiles = (uint32_t)ut.size(); n_ltiles = (uint32_t)lt.size(); n_stiles = (uint32_t)st.size();
+        n_utiles = (uint32_t)ut.size(); n_ftiles = (uint32_t)ft.size();
+        n_ltiles = (uint32_t)lt.size(); n_stiles = (uint32_t)st.size();
         auto up = [&](void* d, const void* h, size_t n) { return hip(hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, stream), "hipMemcpyAsync(tables)"); };
         if ((r = up(utiles, ut.data(), sizeof(uint32_t)*ut.size()))) return r;
+        if ((r = up(ftiles, ft.data(), sizeof(uint32_t)*ft.size()))) return r;
         if ((r = up(ltiles, lt.data(), sizeof(uint32_t)*lt.size()))) return r;
         if ((r = up(stiles, st.data(), sizeof(uint32_t)*st.size()))) return r;
         if ((r = up(ens_dev, ens.data(), sizeof(BatchEns)*count))) return r;
@@ -191,7 +203,8 @@ struct tcym_batch : tcym::DeviceQueue {
         if ((r = up(red_ln, ln.data(), sizeof(uint32_t)*count))) return r;
         if ((r = up(red_s0, s0.data(), sizeof(uint32_t)*count))) return r;
         if ((r = up(red_sn, sn.data(), sizeof(uint32_t)*count))) return r;
-        shape = shape_;
+        shape[0] = su;
+        shape[1] = sf;
         drop_graphs();
         return sync();
     }
@@ -307,7 +320,7 @@ int tcym_batch_create(const tcym_batch_params* p, tcym_batch** out) {
     if (!r) r = b->hip(hipHostMalloc((void**)&b->red_host, sizeof(double)*6*(size_t)b->count, hipHostMallocDefault), "hipHostMalloc");
     if (!r) r = b->alloc((void**)&b->stage, sizeof(double)*(size_t)b->max_link_host(), "hipMalloc(stage)");
     if (!r) r = b->alloc((void**)&b->counters, sizeof(uint32_t)*4, "hipMalloc(counters)");
-    if (!r) r = b->build_tables(-1);
+    if (!r) r = b->build_tables(-1, -1);
     if (!r) r = b->sync();
     if (r) { tcym_batch_destroy(b); return r; }
     *out = b;
@@ -598,44 +611,62 @@ int tcym_batch_unitarity(tcym_batch* b, int u, double* out) {
 }
 
 // ---------------------------------------------------------------- autotuner
+// Times every candidate walk shape x persistent grid of one pass on scratch
+// copies of the fields and keeps the fastest for that pass.  Leapfrog and
+// Langevin (mode 0, 1): reps launches on the same input.  Flow (mode 3): the
+// cadence of a cooling episode, 4 reps steps that alternate between two link
+// buffers with columns 0 and 1 stored between them.
 int tcym_batch_tune(tcym_batch* b, int mode, double dt, int reps, int choice[3], double* us) {
     if (!b) return TCYM_E_ARG;
     if (mode != 0 && mode != 1 && mode != 3) return b->fail(TCYM_E_ARG, "tune: mode 0, 1 or 3");
     if (reps < 1) reps = 3;
+    const int pass = mode == 3 ? 1 : 0;
     int r;
-    if ((r = b->useU(TCYM_U_HOT)) || (r = b->useE(TCYM_E_HOT)) || (r = b->useE(TCYM_E_AUX)) || (r = b->scratch())) return r;
-    // candidates run on copies: U_HOT -> scratch, E_AUX (a copy of E_HOT) in place
+    if ((r = b->useU(TCYM_U_HOT)) || (r = b->useU(TCYM_U_AUX)) || (r = b->useE(TCYM_E_HOT)) || (r = b->useE(TCYM_E_AUX)) ||
+        (r = b->scratch()))
+        return r;
+    // candidates run on copies: E_AUX (a copy of E_HOT) in place; links
+    // U_HOT -> scratch (leapfrog, Langevin) or U_AUX <-> scratch (flow)
     b->be->copy(b->L(), b->e_elems, b->E[TCYM_E_HOT], b->E[TCYM_E_AUX]);
+    b->be->copy(b->L(), b->u_elems, b->U[TCYM_U_HOT], b->U[TCYM_U_AUX]);
     if ((r = b->sync())) return r;
-    const int shape0 = b->shape, per0 = b->per_cu;
-    const bool split0 = b->xcd_split;
+    const int shape0[2] = {b->shape[0], b->shape[1]};
     const bool timing0 = b->timing;
     b->timing = false;
     hipEvent_t e0 = b->ev(), e1 = b->ev();
     double best = 1e300;
-    int bs = shape0, bp = per0;
-    bool bx = split0;
-    std::vector<int> shapes = {-1, 0, 4, 5, 6};
-    for (int s : shapes) {
-        if ((r = b->build_tables(s))) break;
+    int bs = shape0[pass], bp = b->per_cu[pass];
+    bool bx = b->xcd_split[pass];
+    const int nflow = 4*reps;
+    auto run = [&](const tcym::BatchDev& d, int n) {
+        void* a = b->U[TCYM_U_AUX];
+        void* c = b->Uscr;
+        for (int k = 0; k < n; k++) {
+            if (mode == 3) {
+                b->be->update_batch(b->L(), d, 3, false, dt, 0.0, 0.0, 0, a, c, nullptr, nullptr, k == n - 1);
+                std::swap(a, c);
+            } else {
+                b->be->update_batch(b->L(), d, mode, true, dt, 0.1, mode == 1 ? 0.1 : 0.0, 0, b->U[TCYM_U_HOT], b->Uscr,
+                                    b->E[TCYM_E_AUX], nullptr, false);
+            }
+        }
+    };
+    for (int s : {-1, 0, 4, 5, 6}) {
+        if ((r = pass == 0 ? b->build_tables(s, shape0[1]) : b->build_tables(shape0[0], s))) break;
         for (int pc : {2, 4, 8, 16}) {
             for (bool xs : {true, false}) {
                 if (!xs && pc != 8) continue;
-                b->per_cu = pc;
-                b->xcd_split = xs;
+                b->per_cu[pass] = pc;
+                b->xcd_split[pass] = xs;
                 const tcym::BatchDev d = b->dev();
-                // one warm-up launch, then reps timed launches
-                b->be->update_batch(b->L(), d, mode, mode != 3, dt, 0.1, mode == 1 ? 0.1 : 0.0, 0,
-                                    b->U[TCYM_U_HOT], b->Uscr, b->E[TCYM_E_AUX], nullptr, false);
+                run(d, mode == 3 ? 2 : 1);     // warm-up
                 hipEventRecord(e0, b->stream);
-                for (int k = 0; k < reps; k++)
-                    b->be->update_batch(b->L(), d, mode, mode != 3, dt, 0.1, mode == 1 ? 0.1 : 0.0, 0,
-                                        b->U[TCYM_U_HOT], b->Uscr, b->E[TCYM_E_AUX], nullptr, false);
+                run(d, mode == 3 ? nflow : reps);
                 hipEventRecord(e1, b->stream);
                 if ((r = b->sync())) break;
                 float ms = 0.f;
                 hipEventElapsedTime(&ms, e0, e1);
-                const double t = 1e3*ms/reps;
+                const double t = 1e3*ms/(mode == 3 ? nflow : reps);
                 if (t < best) { best = t; bs = s; bp = pc; bx = xs; }
             }
             if (r) break;
@@ -646,9 +677,9 @@ int tcym_batch_tune(tcym_batch* b, int mode, double dt, int reps, int choice[3],
     b->free_events.push_back(e1);
     b->timing = timing0;
     if (r) return r;
-    b->per_cu = bp;
-    b->xcd_split = bx;
-    if ((r = b->build_tables(bs))) return r;
+    b->per_cu[pass] = bp;
+    b->xcd_split[pass] = bx;
+    if ((r = pass == 0 ? b->build_tables(bs, shape0[1]) : b->build_tables(shape0[0], bs))) return r;
     if (choice) { choice[0] = bs; choice[1] = bp; choice[2] = bx ? 1 : 0; }
     if (us) *us = best;
     return TCYM_OK;

@@ -108,15 +108,35 @@ template <typename T> struct SU2 {
     }
     __device__ static __forceinline__ T retr_1m(const Mat& M) { return T(2)*(T(1) - M.v[3]); }
 
-    // exp(i c al^a sigma_a/2) in closed form
+    // exp(i c al^a sigma_a/2) = cos h + i (c sin(h)/(2h)) al.sigma, h = |c al|/2.
+    // FP32: cos h and sin(h)/h as series in s = h^2; for the small angles of
+    // the update steps (s <= 0.6) the truncated series is below FP32 rounding
+    // and needs no range reduction, square root or division.  FP64 and larger
+    // angles: the library's sin and cos (the FP64 form follows the oracle's
+    // arithmetic to the last bits that the Gauss-law restoration's stopping
+    // test can see).
     __device__ static __forceinline__ Mat expi(T c, const T* al) {
         T x0 = c*al[0], x1 = c*al[1], x2 = c*al[2];
-        T d = dsqrt(x0*x0 + x1*x1 + x2*x2);
+        const T dd = x0*x0 + x1*x1 + x2*x2;
+        const T s = T(0.25)*dd;
+        constexpr int KC = 5, KS = 4;     // last series terms s^K
         Mat U;
-        if (d != T(0)) {
-            T h = T(0.5)*d, m = c*dsin(h)/d;
+        if (sizeof(T) == 4 && s <= T(0.6)) {
+            // Horner: cos h = sum_k (-1)^k s^k/(2k)!, sin(h)/h = sum_k (-1)^k s^k/(2k+1)!
+            T ch = T((KC & 1) ? -kInvFact[2*KC] : kInvFact[2*KC]);
+#pragma unroll
+            for (int k = KC - 1; k >= 0; k--) ch = fmat(ch, s, T((k & 1) ? -kInvFact[2*k] : kInvFact[2*k]));
+            T sh = T((KS & 1) ? -kInvFact[2*KS + 1] : kInvFact[2*KS + 1]);
+#pragma unroll
+            for (int k = KS - 1; k >= 0; k--) sh = fmat(sh, s, T((k & 1) ? -kInvFact[2*k + 1] : kInvFact[2*k + 1]));
+            const T m = T(0.5)*c*sh;
+            U.v[0] = m*al[0]; U.v[1] = m*al[1]; U.v[2] = m*al[2]; U.v[3] = ch;
+        } else if (dd != T(0)) {
+            const T d = dsqrt(dd), h = T(0.5)*d, m = c*dsin(h)/d;
             U.v[0] = m*al[0]; U.v[1] = m*al[1]; U.v[2] = m*al[2]; U.v[3] = dcos(h);
-        } else U = unit();
+        } else {
+            U = unit();
+        }
         return U;
     }
     // al with exp(i c al t) = U, principal branch (atan2)
@@ -691,9 +711,10 @@ struct Philox {
     // Box-Muller on two 53-bit uniforms.  T = double for the FP64 form (and for
     // the initial conditions of both forms); T = float evaluates the logarithm
     // and the sine/cosine in FP32 from the same uniforms (a tiny u1 keeps its
-    // magnitude in FP32, so the tails are those of the FP64 draw) at a quarter
-    // of the cost; the FP32 form's noise then differs from the FP64 form's by
-    // FP32 rounding.
+    // magnitude in FP32, so the tails are those of the FP64 draw; sine and
+    // cosine of 2 pi u2 come from sincospif, whose argument reduction is exact)
+    // at a fraction of the cost; the FP32 form's noise then differs from the
+    // FP64 form's by FP32 rounding.
     template <typename T = double>
     __device__ static __forceinline__ void normals(uint64_t seed, uint32_t stream, uint32_t step, uint32_t link, int n, T* z) {
         for (int j = 0; 2*j < n; j++) {
@@ -705,8 +726,9 @@ struct Philox {
                 R = __builtin_sqrt(-2.0*log(u1));
                 sincos(6.283185307179586476925286766559*u2, &sn, &cs);
             } else {
+                // sin, cos of 2 pi u2 as sin, cos of pi (2 u2): the reduction is exact
                 R = __builtin_sqrtf(-2.0f*logf((float)u1));
-                sincosf(6.2831853071795864769f*(float)u2, &sn, &cs);
+                sincospif(2.0f*(float)u2, &sn, &cs);
             }
             z[2*j] = R*cs;
             if (2*j + 1 < n) z[2*j + 1] = R*sn;
