@@ -90,11 +90,13 @@ def make_initial_condition(T, nc, N, seed, su, se, maxiter):
     return U, E, it, fm
 
 
-def run_form(T, nc, N, precision, U, E, steps, warmup, dt, cool_every, cool_steps, dtau, timing):
+def run_form(T, nc, N, precision, U, E, steps, warmup, dt, cool_every, cool_steps, dtau, timing, form=0):
     """`steps` measurement intervals (cool_every leapfrog steps, a cool_steps
     cooling episode, the geodesic Delta N_CS and the energy) after the initial
-    cooling, all timed; `warmup` untimed intervals run first on copies."""
+    cooling, all timed; `warmup` untimed intervals run first on copies.
+    `form`: the update kernel form (T.FORM_AUTO or T.FORM_LINK)."""
     g = T.Solver(nc, N, precision)
+    g.set_update_form(form)
     g.set_links(U)
     g.set_efield(E)
     # warm-up on copies of the fields: the same calls as a measurement interval
@@ -130,16 +132,22 @@ def run_form(T, nc, N, precision, U, E, steps, warmup, dt, cool_every, cool_step
                 % (k, n_lf, energies[-1], ncs, time.perf_counter() - t0))
     g.sync()
     wall = time.perf_counter() - t0
-    stats = {}
+    stats, forms = {}, {}
     if timing:
         for name in ("leapfrog", "flow", "edotb", "geodesic", "copy", "bulk", "reduce"):
             stats[name] = g.kernel_time(name)
+        # the update kernels of the plaquette form are timed under their own names
+        for name in ("leapfrog", "flow"):
+            ms, n = g.kernel_time(name + "_plaq")
+            forms[name] = "plaquette" if n else "link"
+            if n:
+                stats[name] = (stats[name][0] + ms, stats[name][1] + n)
         g.timing(False)
     umax = g.unitarity_violation()[0]
     gauss = g.gauss_violation()
     g.close()
     return dict(wall=wall, n_lf=n_lf, n_flow=n_flow, energies=energies, ncs=dncs_series,
-                stats=stats, unitarity=umax, gauss=gauss)
+                stats=stats, forms=forms, unitarity=umax, gauss=gauss)
 
 
 def batched_sizes(count, seed):
@@ -276,6 +284,8 @@ def main():
     ap.add_argument("--seed", type=int, default=5005)
     ap.add_argument("--restore-maxiter", type=int, default=10000)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 form")
+    ap.add_argument("--no-link-form", action="store_true",
+                    help="skip the FP32 run with the per-link update kernels (the comparison of the two forms)")
     ap.add_argument("--cpu-seconds", type=float, default=16.0, help="CPU baseline budget per run (0 = skip)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensembles workload")
     ap.add_argument("--batch-count", type=int, default=64)
@@ -297,7 +307,13 @@ def main():
     r32 = run_form(T, nc, N, T.FP32, U, E, args.steps, args.warmup, args.dt, args.cool_every,
                    args.cool_steps, args.dtau, timing=True)
     lu32 = 3 * V * (r32["n_lf"] + r32["n_flow"]) / r32["wall"]
-    log("# FP32 form: %.3f s, %.3e link-updates/s" % (r32["wall"], lu32))
+    log("# FP32 form: %.3f s, %.3e link-updates/s (update kernels: %s)" % (r32["wall"], lu32, r32["forms"]))
+    r32l = None
+    if not args.no_link_form and "plaquette" in r32["forms"].values():
+        r32l = run_form(T, nc, N, T.FP32, U, E, args.steps, args.warmup, args.dt, args.cool_every,
+                        args.cool_steps, args.dtau, timing=True, form=T.FORM_LINK)
+        log("# FP32 form, per-link update kernels: %.3f s, %.3e link-updates/s"
+            % (r32l["wall"], 3 * V * (r32l["n_lf"] + r32l["n_flow"]) / r32l["wall"]))
     r64 = None
     if not args.no_fp64:
         r64 = run_form(T, nc, N, T.FP64, U, E, args.steps, args.warmup, args.dt, args.cool_every,
@@ -317,14 +333,14 @@ def main():
     for k, (ms, n) in r32["stats"].items():
         if n:
             kernels[k] = {"launches": n, "avg_us": 1e3 * ms / n}
-    for fk, rr in (("fp32", r32), ("fp64", r64)):
+    for fk, rr in (("fp32", r32), ("fp32_link_form", r32l), ("fp64", r64)):
         if rr is None:
             continue
         for kk in ("leapfrog", "flow"):
             ms, n = rr["stats"][kk]
             if n:
-                b = bytes_per_link_update(nc, kk, fk == "fp32")
-                kernels["%s_%s" % (kk, fk)] = {"avg_us": 1e3 * ms / n, "launches": n,
+                b = bytes_per_link_update(nc, kk, fk != "fp64")
+                kernels["%s_%s" % (kk, fk)] = {"avg_us": 1e3 * ms / n, "launches": n, "form": rr["forms"].get(kk),
                                                "GB/s": 3 * V * b / (ms / n / 1e3) / 1e9,
                                                "LU/s": 3 * V / (ms / n / 1e3)}
 
@@ -350,7 +366,7 @@ def main():
             "gauss_restore_iterations": it, "gauss_restore_signed_max": fm,
         },
         "roofline": {
-            "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "bound": "hbm", "kernel": "%s (%s form)" % (dom, r32["forms"].get(dom)), "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_algorithmic": 3 * V * bpl, "traffic_source": traffic_src,
             "bytes_per_link_update": bpl, "avg_launch_us": avg_s * 1e6,
@@ -364,6 +380,14 @@ def main():
                  "gauss_signed_max": r32["gauss"][0], "gauss_rms": r32["gauss"][1],
                  "delta_ncs": r32["ncs"][-1] if r32["ncs"] else None},
     }
+    if r32l is not None:
+        lul = 3 * V * (r32l["n_lf"] + r32l["n_flow"]) / r32l["wall"]
+        out["fp32_link_form"] = {
+            "wall_s": r32l["wall"], "link_updates_per_s": lul, "plaquette_form_speedup": lu32 / lul,
+            "max_abs_delta_ncs_diff_vs_plaquette_form": max(abs(a - b) for a, b in zip(r32["ncs"], r32l["ncs"])),
+            "max_rel_energy_diff_vs_plaquette_form": max(abs(a - b) / abs(b) for a, b in zip(r32["energies"],
+                                                                                           r32l["energies"])),
+        }
     if r64 is not None:
         e32, e64 = r32["energies"], r64["energies"]
         drift = max(abs(a - b) / abs(b) for a, b in zip(e32, e64)) if e64 else None

@@ -244,6 +244,19 @@ int tcym_delta_ncs_cooling(tcym_ctx *ctx, int uold, int unew, double *out);
 /* ChernSimonsNumber::VacuumEstimator::NCS (TOPOLOGY/VacuumEstimator.cpp:7-86) */
 int tcym_vacuum_ncs(tcym_ctx *ctx, int u, double *out);
 
+/* ---- update kernel form ---- */
+/* Form of the leapfrog, Langevin and flow update kernels (no reference
+   counterpart: the reference has one CPU loop).  TCYM_FORM_AUTO multiplies
+   out each plaquette once and shares it among its four links where the
+   lattice allows it (FP32 storage, N[0] a multiple of 64 and at most 384,
+   N[1] with a divisor PY >= 2, N[0] PY <= 768), else computes the staples of
+   every link apart; TCYM_FORM_LINK always does the latter.  The two forms
+   sum the same force terms in another order (results agree to FP32
+   rounding); the batch kernels use the per-link form. */
+#define TCYM_FORM_AUTO 0
+#define TCYM_FORM_LINK 1
+int tcym_set_update_form(tcym_ctx *ctx, int form);
+
 /* ---- measurement support ---- */
 /* per-kernel HIP-event timing on the context's stream */
 int tcym_timing(tcym_ctx *ctx, int enable);

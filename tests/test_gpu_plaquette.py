@@ -1,0 +1,122 @@
+"""The plaquette form of the update kernels (k_update_plaq: each plaquette
+multiplied out once and shared among its four links, FP32 storage, lattices
+whose x extent is a multiple of 64) against the oracle and against the
+per-link form (TCYM_FORM_LINK) on the same seeded inputs.
+
+Both forms sum the same force terms in FP32, in different orders, so they
+agree with each other and with the FP64 oracle to FP32 rounding (atol 5e-5
+after a few steps, the bar of the per-link form's own FP32 tests); the
+plaquette form must not be further from the oracle than the per-link form
+is.  Shapes cover one block, several blocks along y and z (the halo row and
+the z slab boundaries, periodic wrap included), a z extent that the slab
+length does not divide, nx = 128 and 192, and anisotropic spacings."""
+import numpy as np
+import pytest
+
+import _oracle as O
+import topologicalcymsolver_amd as T
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(64, 8, 8), (64, 16, 24), (64, 7, 6), (128, 4, 5), (192, 8, 13)]
+SPACINGS = [((1.0, 1.0, 1.0), 1.0), ((1.0, 1.3, 0.7), 0.9)]
+KERNEL = {"leapfrog": "leapfrog", "langevin": "langevin", "flow": "flow"}
+
+
+def run(nc, N, a, aScale, kind, form, lat0, steps):
+    g = T.Solver(nc, N, T.FP32, a=a, aScale=aScale)
+    g.set_update_form(form)
+    g.set_links(lat0.U)
+    g.set_efield(lat0.E)
+    g.timing(True)
+    if kind == "leapfrog":
+        g.dynamics_update(steps, 0.05)
+    elif kind == "langevin":
+        g.langevin_update(steps, 0.05, 0.5, 0.4, 77, 3)
+    else:
+        g.gradient_flow(steps, 0.02, u=T.U_HOT, e=T.E_HOT)
+    plaq = g.kernel_time(KERNEL[kind] + "_plaq")[1]
+    link = g.kernel_time(KERNEL[kind])[1]
+    out = g.get_links(), g.get_efield(), g.energy(), g.ncs_dot()
+    g.close()
+    return out, plaq, link
+
+
+def oracle(nc, N, a, aScale, kind, lat0, steps):
+    lat = lat0.copy()
+    if kind == "leapfrog":
+        lat.leapfrog(steps, 0.05)
+    elif kind == "langevin":
+        lat.langevin(steps, 0.05, 0.5, 0.4, 77, 3)
+    else:
+        lat.flow(steps, 0.02)
+    return lat
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+@pytest.mark.parametrize("N", SHAPES)
+@pytest.mark.parametrize("kind", ["leapfrog", "langevin", "flow"])
+@pytest.mark.parametrize("spacing", SPACINGS)
+def test_plaquette_form_matches_oracle_and_link_form(nc, N, kind, spacing):
+    a, aScale = spacing
+    steps = 4
+    print("plaquette form: SU(%d) %s %s a=%s aScale=%g" % (nc, N, kind, a, aScale), flush=True)
+    lat0 = O.Lattice(nc, N, a, aScale)
+    lat0.random(31, 0.3, 0.3, restore=False)
+    ref = oracle(nc, N, a, aScale, kind, lat0, steps)
+    (Up, Ep, enp, ndp), plaq_p, link_p = run(nc, N, a, aScale, kind, T.FORM_AUTO, lat0, steps)
+    (Ul, El, enl, ndl), plaq_l, link_l = run(nc, N, a, aScale, kind, T.FORM_LINK, lat0, steps)
+    # the plaquette kernel ran in the default form and only there
+    assert (plaq_p, link_p) == (steps, 0)
+    assert (plaq_l, link_l) == (0, steps)
+    errp = max(np.abs(Up - ref.U).max(), np.abs(Ep - ref.E).max())
+    errl = max(np.abs(Ul - ref.U).max(), np.abs(El - ref.E).max())
+    assert errp < 5e-5 and errl < 5e-5, (errp, errl)
+    assert errp <= 2.0 * errl + 1e-6, (errp, errl)
+    assert np.abs(Up - Ul).max() < 5e-5 and np.abs(Ep - El).max() < 5e-5
+    en = ref.energy()
+    assert abs(enp - en) < 1e-5 * abs(en) and abs(enl - en) < 1e-5 * abs(en)
+    nd = ref.ncsdot()
+    assert abs(ndp - nd) < 1e-5 and abs(ndl - nd) < 1e-5
+
+
+def test_plaquette_form_long_flow_tracks_link_form():
+    """200 flow steps (a cooling episode) at SU(3) 192x8x13: the two forms stay
+    within FP32 rounding of each other and both keep the links unitary."""
+    N = (192, 8, 13)
+    lat0 = O.Lattice(3, N)
+    lat0.random(5, 0.3, 0.3, restore=False)
+    res = []
+    for form in (T.FORM_AUTO, T.FORM_LINK):
+        g = T.Solver(3, N, T.FP32)
+        g.set_update_form(form)
+        g.set_links(lat0.U)
+        g.set_efield(lat0.E)
+        g.gradient_flow(200, 0.02, u=T.U_HOT, e=T.E_HOT)
+        assert g.unitarity_violation()[0] < 2e-6
+        res.append((g.get_links(), g.energy()))
+        g.close()
+    assert np.abs(res[0][0] - res[1][0]).max() < 1e-4
+    assert abs(res[0][1] - res[1][1]) < 1e-5 * abs(res[1][1])
+
+
+def test_update_form_argument_checked():
+    g = T.Solver(3, (64, 4, 4), T.FP32)
+    with pytest.raises(T.TcymError):
+        g.set_update_form(7)
+    g.close()
+    # shapes the plaquette form does not take run the per-link kernel
+    for N in [(48, 8, 8), (64, 13, 8), (448, 4, 4)]:
+        g = T.Solver(3, N, T.FP32)
+        g.timing(True)
+        g.random_links(3, 0.2)
+        g.dynamics_update(1, 0.05)
+        assert g.kernel_time("leapfrog")[1] == 1 and g.kernel_time("leapfrog_plaq")[1] == 0
+        g.close()
+    # FP64 storage: per-link form
+    g = T.Solver(3, (64, 8, 8), T.FP64)
+    g.timing(True)
+    g.random_links(3, 0.2)
+    g.dynamics_update(1, 0.05)
+    assert g.kernel_time("leapfrog")[1] == 1 and g.kernel_time("leapfrog_plaq")[1] == 0
+    g.close()

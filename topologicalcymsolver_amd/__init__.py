@@ -18,6 +18,7 @@ LIB_PATH = os.environ.get("TCYM_LIBRARY") or os.path.join(PKG, "libtcym.so")
 
 OK, E_ARG, E_HIP, E_NOCONV, E_NONFINITE, E_UNSUPPORTED, E_IO = 0, -1, -2, -3, -4, -5, -6
 FP64, FP32 = 0, 1
+FORM_AUTO, FORM_LINK = 0, 1
 U_HOT, U_COOL, U_OLD, U_NEW, U_MID, U_AUX = range(6)
 E_HOT, E_COOL, E_MID, E_XI, E_AUX = range(5)
 
@@ -33,7 +34,7 @@ EXPORTS = [
     "tcym_gauss_restore", "tcym_geodesic", "tcym_delta_ncs_cooling", "tcym_vacuum_ncs",
     "tcym_block", "tcym_level", "tcym_cool_blocked", "tcym_cool_continue", "tcym_continue_cooling", "tcym_thermalize",
     "tcym_quasi_particles",
-    "tcym_timing", "tcym_kernel_time", "tcym_timing_reset", "tcym_sync", "tcym_device_bytes",
+    "tcym_set_update_form", "tcym_timing", "tcym_kernel_time", "tcym_timing_reset", "tcym_sync", "tcym_device_bytes",
     "tcym_get_params",
 ]
 # every exported symbol of include/tcym_io.h
@@ -115,7 +116,7 @@ def load_library(path=LIB_PATH):
         "tcym_hard_scales": [V, I, _dp],
         "tcym_gauss_restore": [V, I, I, I, D, ctypes.POINTER(I), _dp],
         "tcym_geodesic": [V, I, I, I, I, D], "tcym_delta_ncs_cooling": [V, I, I, _dp],
-        "tcym_vacuum_ncs": [V, I, _dp], "tcym_timing": [V, I],
+        "tcym_vacuum_ncs": [V, I, _dp], "tcym_timing": [V, I], "tcym_set_update_form": [V, I],
         "tcym_block": [V, I, I], "tcym_level": [V, I, I, ctypes.POINTER(I), ctypes.POINTER(I)],
         "tcym_cool_blocked": [V, I, I, I, D, I, I, ctypes.POINTER(I)],
         "tcym_cool_continue": [V, I, I, I, D, I, I, ctypes.POINTER(I)],
@@ -509,6 +510,11 @@ class Solver:
         self._ck(self.L.tcym_get_params(self.h, ctypes.byref(p)))
         return {"nc": p.nc, "precision": p.precision, "N": tuple(p.N), "a": tuple(p.a), "aScale": p.aScale,
                 "device": p.device}
+
+    def set_update_form(self, form):
+        """FORM_AUTO: plaquettes shared among their links where the lattice
+        allows it; FORM_LINK: the staples of every link apart."""
+        self._ck(self.L.tcym_set_update_form(self.h, form))
 
     def timing(self, enable=True):
         self._ck(self.L.tcym_timing(self.h, 1 if enable else 0))

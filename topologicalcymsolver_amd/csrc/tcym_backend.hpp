@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <functional>
 
+#include "tcym.h"
+
 namespace tcym {
 
 struct Geom {
@@ -12,6 +14,7 @@ struct Geom {
     long V;            // sites
     double a[3];       // lattice spacings
     double aScale;
+    int form;          // update kernel form (TCYM_FORM_*)
 };
 
 // one kernel launch; the context wraps it with optional HIP-event timing

@@ -157,6 +157,7 @@ template <typename T> struct SU2 {
         o[2] = s*E[2] + T(2)*V.v[2]*ve + T(2)*w*x2;
     }
     __device__ static __forceinline__ void inv_adj_act(const Mat& V, const T* E, T* o) { adj_act(dag(V), E, o); }
+    __device__ static __forceinline__ void ad_dag(const Mat& V, const T* f, T* o) { adj_act(dag(V), f, o); }
     __device__ static __forceinline__ T unitarity_norm(const Mat& U) {
         Mat C = mul_nd(U, U);
         return dsqrt(C.v[0]*C.v[0] + C.v[1]*C.v[1] + C.v[2]*C.v[2] + (C.v[3] - T(1))*(C.v[3] - T(1)));
@@ -388,6 +389,51 @@ template <typename T> struct SU3 {
         Mat A = alg_matrix(T(1), E);
         Mat X = mul(mul_dn(V, A), V);
         herm_alg(X, o);
+    }
+    // o = (V^+ H V)^a, H = f^a t^a, for special unitary V: Y = H V whole,
+    // then of the hermitian V^+ Y only the diagonal and the lower triangle
+    // that herm_alg reads (its upper triangle is their conjugate)
+    __device__ static __forceinline__ void ad_dag(const Mat& V, const T* f, T* o) {
+        const Mat H = alg_matrix(T(1), f);
+        Mat Y;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                // H(r, r) is real
+                T sr = H.r[r + 3*r]*V.r[r + 3*c], si = H.r[r + 3*r]*V.i[r + 3*c];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    if (k == r) continue;
+                    sr = fmat(H.r[r + 3*k], V.r[k + 3*c], sr); sr = fmat(-H.i[r + 3*k], V.i[k + 3*c], sr);
+                    si = fmat(H.r[r + 3*k], V.i[k + 3*c], si); si = fmat(H.i[r + 3*k], V.r[k + 3*c], si);
+                }
+                Y.r[r + 3*c] = sr; Y.i[r + 3*c] = si;
+            }
+        }
+        // X(r, c) = sum_k conj(V(k, r)) Y(k, c)
+        auto xre = [&](int r, int c) {
+            T s = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; k++) { s = fmat(V.r[k + 3*r], Y.r[k + 3*c], s); s = fmat(V.i[k + 3*r], Y.i[k + 3*c], s); }
+            return s;
+        };
+        auto xim = [&](int r, int c) {
+            T s = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; k++) { s = fmat(V.r[k + 3*r], Y.i[k + 3*c], s); s = fmat(-V.i[k + 3*r], Y.r[k + 3*c], s); }
+            return s;
+        };
+        const T s3 = T(0.57735026918962576450914878050196);
+        const T d0 = xre(0, 0), d1 = xre(1, 1), d2 = xre(2, 2);
+        o[0] = T(2)*xre(1, 0);
+        o[1] = T(2)*xim(1, 0);
+        o[2] = d0 - d1;
+        o[3] = T(2)*xre(2, 0);
+        o[4] = T(2)*xim(2, 0);
+        o[5] = T(2)*xre(2, 1);
+        o[6] = T(2)*xim(2, 1);
+        o[7] = (d0 + d1 - T(2)*d2)*s3;
     }
 
     // exp(i Q), Q = c al^a t^a, as f0 + f1 Q + f2 Q^2 (Cayley-Hamilton):
