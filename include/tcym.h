@@ -246,15 +246,21 @@ int tcym_vacuum_ncs(tcym_ctx *ctx, int u, double *out);
 
 /* ---- update kernel form ---- */
 /* Form of the leapfrog, Langevin and flow update kernels (no reference
-   counterpart: the reference has one CPU loop).  TCYM_FORM_AUTO multiplies
-   out each plaquette once and shares it among its four links where the
-   lattice allows it (FP32 storage, N[0] a multiple of 64 and at most 384,
-   N[1] with a divisor PY >= 2, N[0] PY <= 768), else computes the staples of
-   every link apart; TCYM_FORM_LINK always does the latter.  The two forms
+   counterpart: the reference has one CPU loop).  The plaquette form
+   multiplies out each plaquette once and shares it among its four links; it
+   takes FP32 storage and lattices with N[0] a multiple of 64 and at most
+   384 and N[1] with a divisor PY >= 2, N[0] PY <= 768.  The per-link form
+   computes the staples of every link apart.  TCYM_FORM_AUTO (the default)
+   runs the plaquette form where it applies and was measured faster (its
+   grid fills the device with at least two blocks per compute unit, and for
+   SU(3) N[0] >= 192: SU(3) 192^3 and 256^3, SU(2) 128^3 and up), else the
+   per-link form; TCYM_FORM_LINK always the per-link form;
+   TCYM_FORM_PLAQUETTE the plaquette form wherever it applies.  The two forms
    sum the same force terms in another order (results agree to FP32
    rounding); the batch kernels use the per-link form. */
-#define TCYM_FORM_AUTO 0
-#define TCYM_FORM_LINK 1
+#define TCYM_FORM_AUTO      0
+#define TCYM_FORM_LINK      1
+#define TCYM_FORM_PLAQUETTE 2
 int tcym_set_update_form(tcym_ctx *ctx, int form);
 
 /* ---- measurement support ---- */

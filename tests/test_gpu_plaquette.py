@@ -64,9 +64,9 @@ def test_plaquette_form_matches_oracle_and_link_form(nc, N, kind, spacing):
     lat0 = O.Lattice(nc, N, a, aScale)
     lat0.random(31, 0.3, 0.3, restore=False)
     ref = oracle(nc, N, a, aScale, kind, lat0, steps)
-    (Up, Ep, enp, ndp), plaq_p, link_p = run(nc, N, a, aScale, kind, T.FORM_AUTO, lat0, steps)
+    (Up, Ep, enp, ndp), plaq_p, link_p = run(nc, N, a, aScale, kind, T.FORM_PLAQUETTE, lat0, steps)
     (Ul, El, enl, ndl), plaq_l, link_l = run(nc, N, a, aScale, kind, T.FORM_LINK, lat0, steps)
-    # the plaquette kernel ran in the default form and only there
+    # the plaquette kernel ran in the plaquette form and only there
     assert (plaq_p, link_p) == (steps, 0)
     assert (plaq_l, link_l) == (0, steps)
     errp = max(np.abs(Up - ref.U).max(), np.abs(Ep - ref.E).max())
@@ -87,7 +87,7 @@ def test_plaquette_form_long_flow_tracks_link_form():
     lat0 = O.Lattice(3, N)
     lat0.random(5, 0.3, 0.3, restore=False)
     res = []
-    for form in (T.FORM_AUTO, T.FORM_LINK):
+    for form in (T.FORM_PLAQUETTE, T.FORM_LINK):
         g = T.Solver(3, N, T.FP32)
         g.set_update_form(form)
         g.set_links(lat0.U)
@@ -108,6 +108,7 @@ def test_update_form_argument_checked():
     # shapes the plaquette form does not take run the per-link kernel
     for N in [(48, 8, 8), (64, 13, 8), (448, 4, 4)]:
         g = T.Solver(3, N, T.FP32)
+        g.set_update_form(T.FORM_PLAQUETTE)
         g.timing(True)
         g.random_links(3, 0.2)
         g.dynamics_update(1, 0.05)
@@ -119,4 +120,19 @@ def test_update_form_argument_checked():
     g.random_links(3, 0.2)
     g.dynamics_update(1, 0.05)
     assert g.kernel_time("leapfrog")[1] == 1 and g.kernel_time("leapfrog_plaq")[1] == 0
+    g.close()
+
+
+@pytest.mark.parametrize("nc,N,plaq", [(3, (64, 64, 64), False), (3, (64, 16, 24), False), (3, (192, 192, 192), True),
+                                       (3, (128, 128, 128), False), (2, (128, 128, 128), True), (2, (64, 64, 64), False)])
+def test_auto_form_takes_the_plaquette_form_where_it_is_faster(nc, N, plaq):
+    """TCYM_FORM_AUTO: the plaquette kernel only where at least two of its
+    blocks per compute unit exist (SU(3) 192^3: 768 blocks) and, for SU(3),
+    the x rows are at least 192 sites long; elsewhere the per-link kernel."""
+    g = T.Solver(nc, N, T.FP32)
+    g.timing(True)
+    g.unit_links()
+    g.dynamics_update(1, 0.05)
+    assert g.kernel_time("leapfrog_plaq")[1] == (1 if plaq else 0)
+    assert g.kernel_time("leapfrog")[1] == (0 if plaq else 1)
     g.close()

@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get("TCYM_LIBRARY") or os.path.join(PKG, "libtcym.so")
 
 OK, E_ARG, E_HIP, E_NOCONV, E_NONFINITE, E_UNSUPPORTED, E_IO = 0, -1, -2, -3, -4, -5, -6
 FP64, FP32 = 0, 1
-FORM_AUTO, FORM_LINK = 0, 1
+FORM_AUTO, FORM_LINK, FORM_PLAQUETTE = 0, 1, 2
 U_HOT, U_COOL, U_OLD, U_NEW, U_MID, U_AUX = range(6)
 E_HOT, E_COOL, E_MID, E_XI, E_AUX = range(5)
 
@@ -513,7 +513,8 @@ class Solver:
 
     def set_update_form(self, form):
         """FORM_AUTO: plaquettes shared among their links where the lattice
-        allows it; FORM_LINK: the staples of every link apart."""
+        allows it and fills the device; FORM_LINK: the staples of every link
+        apart; FORM_PLAQUETTE: plaquettes wherever the lattice allows it."""
         self._ck(self.L.tcym_set_update_form(self.h, form))
 
     def timing(self, enable=True):

@@ -902,7 +902,8 @@ int tcym_quasi_particles(tcym_ctx* c, int u, int e, double Qs, double n0, uint64
 // ---------------------------------------------------------------- timing
 int tcym_set_update_form(tcym_ctx* c, int form) {
     if (!c) return TCYM_E_ARG;
-    if (form != TCYM_FORM_AUTO && form != TCYM_FORM_LINK) return c->fail(TCYM_E_ARG, "update form " + std::to_string(form) + ": TCYM_FORM_AUTO or TCYM_FORM_LINK");
+    if (form != TCYM_FORM_AUTO && form != TCYM_FORM_LINK && form != TCYM_FORM_PLAQUETTE)
+        return c->fail(TCYM_E_ARG, "update form " + std::to_string(form) + ": TCYM_FORM_AUTO, _LINK or _PLAQUETTE");
     c->g.form = form;
     return TCYM_OK;
 }
