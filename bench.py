@@ -270,6 +270,105 @@ def cpu_baseline(nc, N, U, E, dt, dtau, budget_s):
                                          % (one["n_lf"], one["n_fl"], one["wall"])})
 
 
+SMALL_CONFIGS = {
+    1: dict(nc=2, L=16, seed=1001, su=0.2, se=0.3, forms=("fp64",), phases=[("leapfrog", 200, 0.05)],
+            text="SU(2) 16^3 FP64, seeded links (sigma 0.2) and E (sigma 0.3) Gauss-projected; 200 leapfrog steps "
+                 "dt=0.05 with the hot N_CS integral (E.B at every step) on the device"),
+    2: dict(nc=2, L=64, seed=2002, su=0.2, se=0.3, forms=("fp32", "fp64"),
+            phases=[("leapfrog", 1000, 0.05), ("langevin", 1000, 0.05)],
+            text="SU(2) 64^3 FP64 vs FP32-mixed, seeded IC as config 1; 1000 leapfrog steps dt=0.05 then 1000 Langevin "
+                 "steps (damping 0.1, T 0.5, Philox noise), hot N_CS integral at every step, energy every 100 steps"),
+    3: dict(nc=3, L=96, seed=3003, su=0.15, se=0.25, forms=("fp32", "fp64"),
+            phases=[("leapfrog", 500, 0.04), ("flow", 500, 0.02)],
+            text="SU(3) 96^3 FP64 vs FP32-mixed, links sigma 0.15, E sigma 0.25 Gauss-projected; 500 leapfrog steps "
+                 "dt=0.04 (hot N_CS integral) then 500 gradient-flow steps dtau=0.02 on a copy"),
+}
+
+
+def run_small(T, cfg, prec, U, E):
+    """One form of a small configuration: the phases in calls of 100 steps,
+    the energy after each call; the wall time of the steps only."""
+    c = SMALL_CONFIGS[cfg]
+    N = (c["L"],) * 3
+    g = T.Solver(c["nc"], N, prec)
+    g.set_links(U)
+    g.set_efield(E)
+    g.sync()
+    ncs, energy, steps = [0.0], [g.energy()], 0
+    wall = 0.0
+    for kind, n, dt in c["phases"]:
+        if kind == "flow":
+            g.copy_links(T.U_COOL, T.U_HOT)
+        done = 0
+        while done < n:
+            k = min(100, n - done)
+            g.sync()
+            t0 = time.perf_counter()
+            if kind == "leapfrog":
+                d = g.evolve_ncs(0, k, dt)
+            elif kind == "langevin":
+                d = g.evolve_ncs(1, k, dt, damping=0.1, T=0.5, seed=77, step0=done)
+            else:
+                g.gradient_flow(k, dt, u=T.U_COOL, e=T.E_COOL)
+                d = 0.0
+            g.sync()
+            wall += time.perf_counter() - t0
+            done += k
+            steps += k
+            if kind != "flow":
+                ncs.append(ncs[-1] + d)
+                energy.append(g.energy())
+    out = dict(wall=wall, lu=3 * g.V * steps / wall, ncs=ncs, energy=energy, steps=steps,
+               gauss=g.gauss_violation()[0], unitarity=g.unitarity_violation()[0])
+    g.close()
+    return out
+
+
+def run_config(T, cfg, cpu_seconds):
+    """Configurations 1-3 (the parity-test cases of the path, timed): one JSON
+    line with the same fields as the default workload's."""
+    c = SMALL_CONFIGS[cfg]
+    N = (c["L"],) * 3
+    log("# config %d: %s" % (cfg, c["text"]))
+    U, E, it, fm = make_initial_condition(T, c["nc"], N, c["seed"], c["su"], c["se"], 10000)
+    res = {}
+    for f in c["forms"]:
+        run_small(T, cfg, T.FP32 if f == "fp32" else T.FP64, U, E)      # warm-up run
+        res[f] = run_small(T, cfg, T.FP32 if f == "fp32" else T.FP64, U, E)
+        log("# %s: %.3f s, %.3e link-updates/s" % (f, res[f]["wall"], res[f]["lu"]))
+    main_form = "fp32" if "fp32" in res else "fp64"
+    r = res[main_form]
+    out = {
+        "metric": "link-updates/s + HBM GB/s, FP32-mixed vs FP64 GPU at |dN_CS|<=1e-3",
+        "value": r["lu"], "unit": "link-updates/s", "n_gpus": 1, "steps": r["steps"], "warmup": 1,
+        "ms_per_step": 1e3 * r["wall"] / r["steps"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if main_form == "fp32" else "f64",
+        "data": "synthetic (seeded Philox initial condition, Gauss-projected)",
+        "config": {"workload": "config %d: %s" % (cfg, c["text"]), "gauss_restore_iterations": it,
+                   "gauss_restore_signed_max": fm},
+    }
+    for f, x in res.items():
+        out[f] = {"wall_s": x["wall"], "link_updates_per_s": x["lu"], "gauss_signed_max": x["gauss"],
+                  "unitarity_max": x["unitarity"], "hot_ncs_final": x["ncs"][-1]}
+    if "fp32" in res and "fp64" in res:
+        a, b = res["fp32"], res["fp64"]
+        out["fp32_vs_fp64"] = {"speedup": a["lu"] / b["lu"],
+                               "max_abs_hot_ncs_diff": max(abs(x - y) for x, y in zip(a["ncs"], b["ncs"])),
+                               "max_rel_energy_diff": max(abs(x - y) / abs(y) for x, y in zip(a["energy"], b["energy"]))}
+    if cpu_seconds > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        smp = cpu_sample(O, c["nc"], N, U, E, c["phases"][0][2], 0.02, cpu_seconds, threads)
+        rate = smp["lf"] if cfg != 3 else 1000.0 / (500.0 / smp["lf"] + 500.0 / smp["fl"])
+        out["cpu_baseline"] = {"value": rate, "unit": "link-updates/s", "cores": threads, "kind": "port",
+                               "cpu": cpu_model(),
+                               "sample": "%d leapfrog + %d flow steps of the oracle (FP64, OpenMP, %d threads, %.1f s)%s"
+                                         % (smp["n_lf"], smp["n_fl"], threads, smp["wall"],
+                                            ", rate weighted 500 : 500" if cfg == 3 else ", leapfrog rate")}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,6 +389,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensembles workload")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--config", type=int, default=5, choices=(1, 2, 3, 5),
+                    help="5: the SU(3) 192^3 job (default; with the batched workload, config 4); 1-3: the smaller "
+                         "configurations of the path, timed alone")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -298,6 +400,9 @@ def main():
         sys.exit(2)
 
     import topologicalcymsolver_amd as T
+    if args.config != 5:
+        run_config(T, args.config, args.cpu_seconds)
+        return
     nc, N = args.nc, (args.N, args.N, args.N)
     V = N[0] * N[1] * N[2]
     log("# SU(%d) %d^3, %d intervals of %d leapfrog steps and a %d-step cooling (+%d warm-up intervals)" %
