@@ -390,50 +390,55 @@ template <typename T> struct SU3 {
         Mat X = mul(mul_dn(V, A), V);
         herm_alg(X, o);
     }
-    // o = (V^+ H V)^a, H = f^a t^a, for special unitary V: Y = H V whole,
-    // then of the hermitian V^+ Y only the diagonal and the lower triangle
-    // that herm_alg reads (its upper triangle is their conjugate)
+    // o = (V^+ H V)^a, H = f^a t^a, for special unitary V.  With H' = 2H (the
+    // Gell-Mann combination f^a lambda_a, entries read off f) and
+    // X' = V^+ H' V: o_1..6 are the real and imaginary parts of the lower
+    // triangle of X', o_3 and o_8 come from its diagonal, and X'(2,2) =
+    // -(X'(0,0) + X'(1,1)) since H' is traceless, so only columns 0 and 1
+    // of Y = H' V are formed (60 instead of 90 multiply-adds)
     __device__ static __forceinline__ void ad_dag(const Mat& V, const T* f, T* o) {
-        const Mat H = alg_matrix(T(1), f);
-        Mat Y;
+        const T s3 = T(0.57735026918962576450914878050196);
+        // H' = f^a lambda_a: real diagonal, lower triangle (1,0), (2,0), (2,1)
+        const T h00 = f[2] + f[7]*s3, h11 = -f[2] + f[7]*s3, h22 = T(-2)*f[7]*s3;
+        const T hr[3][3] = {{h00, f[0], f[3]}, {f[0], h11, f[5]}, {f[3], f[5], h22}};
+        const T hi[3][3] = {{T(0), -f[1], -f[4]}, {f[1], T(0), -f[6]}, {f[4], f[6], T(0)}};
+        T Yr[3][2], Yi[3][2];
 #pragma unroll
         for (int r = 0; r < 3; r++) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                // H(r, r) is real
-                T sr = H.r[r + 3*r]*V.r[r + 3*c], si = H.r[r + 3*r]*V.i[r + 3*c];
+            for (int c = 0; c < 2; c++) {
+                T sr = hr[r][r]*V.r[r + 3*c], si = hr[r][r]*V.i[r + 3*c];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     if (k == r) continue;
-                    sr = fmat(H.r[r + 3*k], V.r[k + 3*c], sr); sr = fmat(-H.i[r + 3*k], V.i[k + 3*c], sr);
-                    si = fmat(H.r[r + 3*k], V.i[k + 3*c], si); si = fmat(H.i[r + 3*k], V.r[k + 3*c], si);
+                    sr = fmat(hr[r][k], V.r[k + 3*c], sr); sr = fmat(-hi[r][k], V.i[k + 3*c], sr);
+                    si = fmat(hr[r][k], V.i[k + 3*c], si); si = fmat(hi[r][k], V.r[k + 3*c], si);
                 }
-                Y.r[r + 3*c] = sr; Y.i[r + 3*c] = si;
+                Yr[r][c] = sr; Yi[r][c] = si;
             }
         }
-        // X(r, c) = sum_k conj(V(k, r)) Y(k, c)
+        // X'(r, c) = sum_k conj(V(k, r)) Y(k, c)
         auto xre = [&](int r, int c) {
             T s = T(0);
 #pragma unroll
-            for (int k = 0; k < 3; k++) { s = fmat(V.r[k + 3*r], Y.r[k + 3*c], s); s = fmat(V.i[k + 3*r], Y.i[k + 3*c], s); }
+            for (int k = 0; k < 3; k++) { s = fmat(V.r[k + 3*r], Yr[k][c], s); s = fmat(V.i[k + 3*r], Yi[k][c], s); }
             return s;
         };
         auto xim = [&](int r, int c) {
             T s = T(0);
 #pragma unroll
-            for (int k = 0; k < 3; k++) { s = fmat(V.r[k + 3*r], Y.i[k + 3*c], s); s = fmat(-V.i[k + 3*r], Y.r[k + 3*c], s); }
+            for (int k = 0; k < 3; k++) { s = fmat(V.r[k + 3*r], Yi[k][c], s); s = fmat(-V.i[k + 3*r], Yr[k][c], s); }
             return s;
         };
-        const T s3 = T(0.57735026918962576450914878050196);
-        const T d0 = xre(0, 0), d1 = xre(1, 1), d2 = xre(2, 2);
-        o[0] = T(2)*xre(1, 0);
-        o[1] = T(2)*xim(1, 0);
-        o[2] = d0 - d1;
-        o[3] = T(2)*xre(2, 0);
-        o[4] = T(2)*xim(2, 0);
-        o[5] = T(2)*xre(2, 1);
-        o[6] = T(2)*xim(2, 1);
-        o[7] = (d0 + d1 - T(2)*d2)*s3;
+        const T d0 = xre(0, 0), d1 = xre(1, 1);
+        o[0] = xre(1, 0);
+        o[1] = xim(1, 0);
+        o[2] = T(0.5)*(d0 - d1);
+        o[3] = xre(2, 0);
+        o[4] = xim(2, 0);
+        o[5] = xre(2, 1);
+        o[6] = xim(2, 1);
+        o[7] = T(0.86602540378443864676372317075294)*(d0 + d1);
     }
 
     // exp(i Q), Q = c al^a t^a, as f0 + f1 Q + f2 Q^2 (Cayley-Hamilton):
