@@ -136,3 +136,39 @@ def test_auto_form_takes_the_plaquette_form_where_it_is_faster(nc, N, plaq):
     assert g.kernel_time("leapfrog_plaq")[1] == (1 if plaq else 0)
     assert g.kernel_time("leapfrog")[1] == (0 if plaq else 1)
     g.close()
+
+
+@pytest.mark.parametrize("nc", [3, 2])
+def test_plaquette_form_langevin_with_noise_field(nc):
+    """Mode 2 (LangevinDynamics::UpdateFields with the Xi field) in the
+    plaquette form: the same step as the in-kernel noise of mode 1 on the
+    noise field that tcym_noise draws for that step, and within FP32 rounding
+    of the per-link form's mode 2."""
+    N = (64, 8, 8)
+    lat0 = O.Lattice(nc, N)
+    lat0.random(41, 0.3, 0.3, restore=False)
+    sd = np.sqrt(2 * 0.5 * 0.4)
+    res = {}
+    for form in (T.FORM_PLAQUETTE, T.FORM_LINK):
+        g = T.Solver(nc, N, T.FP32)
+        g.set_update_form(form)
+        g.set_links(lat0.U)
+        g.set_efield(lat0.E)
+        g.copy_links(T.U_AUX, T.U_HOT)
+        g.copy_efield(T.E_AUX, T.E_HOT)
+        g.timing(True)
+        g.langevin_update(1, 0.05, 0.5, 0.4, seed=5, step0=11)
+        g.random_gaussian_noise(5, 11, sd)
+        g.langevin_update_fields(0.05, 0.5, u=T.U_AUX, e=T.E_AUX)
+        plaq = g.kernel_time("langevin_xi_plaq")[1]
+        assert plaq == (1 if form == T.FORM_PLAQUETTE else 0)
+        np.testing.assert_allclose(g.get_links(T.U_AUX), g.get_links(), atol=2e-6)
+        np.testing.assert_allclose(g.get_efield(T.E_AUX), g.get_efield(), atol=2e-6)
+        res[form] = (g.get_links(T.U_AUX), g.get_efield(T.E_AUX))
+        g.close()
+    np.testing.assert_allclose(res[T.FORM_PLAQUETTE][0], res[T.FORM_LINK][0], atol=1e-5)
+    np.testing.assert_allclose(res[T.FORM_PLAQUETTE][1], res[T.FORM_LINK][1], atol=1e-5)
+    ref = lat0.copy()
+    ref.langevin_xi(ref.noise(5, 11, sd), 0.05, 0.5)
+    np.testing.assert_allclose(res[T.FORM_PLAQUETTE][0], ref.U, atol=5e-5)
+    np.testing.assert_allclose(res[T.FORM_PLAQUETTE][1], ref.E, atol=5e-5)
