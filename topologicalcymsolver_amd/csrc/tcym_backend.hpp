@@ -21,11 +21,27 @@ struct Geom {
 using Launch = std::function<void(hipStream_t)>;
 using Launcher = std::function<void(const char* name, const Launch&)>;
 
+// n / d for 0 <= n < 2^31 as a multiply-high and a shift: m = ceil(2^(31+l) / d)
+// with l = ceil(log2 d) (below 2^32), s = l - 1; n * m / 2^(31+l) exceeds
+// n / d by less than n / 2^31 / d, so the floor is exact.  m = 0 marks d = 1.
+struct FastDiv {
+    uint32_t m, s;
+};
+inline FastDiv fast_div(uint32_t d) {
+    if (d <= 1) return FastDiv{0, 0};
+    uint32_t l = 0;
+    while ((1ull << l) < d) l++;
+    const uint64_t m = ((1ull << (31 + l)) + d - 1)/d;
+    return FastDiv{(uint32_t)m, l - 1};
+}
+
 // geometry as the kernels see it: extents, sites and the walk shape
 struct KGeom {
     int nx, ny, nz;
     long V;
     int pxs, tcx, tcy;   // traversal shape (WalkShape); pxs == 0 is the linear walk
+    FastDiv dnx, dnxy;   // division by nx and nx ny (linear walk)
+    FastDiv dnz, dtcx;   // division by nz and tcx (patch walk)
 };
 
 // One lattice of a batch (device table).  Each pass over a batch has its own
@@ -152,6 +168,12 @@ inline WalkShape patch_shape(const Geom& g, int pxs) {
 }
 inline WalkShape linear_shape(const Geom& g) { return WalkShape{0, 0, 0, (g.V + kBlock - 1)/kBlock}; }
 // the shape the measurement kernels use (and the default of the update kernels)
+inline WalkShape walk_shape(const Geom& g);
+inline KGeom make_kgeom(const Geom& g, const WalkShape& w) {
+    return KGeom{g.n[0], g.n[1], g.n[2], g.V, w.pxs, w.tcx, w.tcy,
+                 fast_div((uint32_t)g.n[0]), fast_div((uint32_t)g.n[0]*(uint32_t)g.n[1]),
+                 fast_div((uint32_t)g.n[2]), fast_div((uint32_t)(w.tcx > 0 ? w.tcx : 1))};
+}
 inline WalkShape walk_shape(const Geom& g) {
     WalkShape best{-1, 0, 0, 0};
     for (int s = 6; s >= 4; s--) {

@@ -51,7 +51,7 @@ __global__ void k_advance(uint32_t* __restrict__ counters, uint32_t steps) {
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[0] += steps;
 }
 
-KGeom kgeom(const Geom& g, const WalkShape& w) { return KGeom{g.n[0], g.n[1], g.n[2], g.V, w.pxs, w.tcx, w.tcy}; }
+KGeom kgeom(const Geom& g, const WalkShape& w) { return make_kgeom(g, w); }
 
 // walk shape of the update kernels: -1 the canonical one, 0 linear, 4..6 patches
 WalkShape update_shape(const Geom& g, int shape) {
