@@ -9,7 +9,8 @@ after a few steps, the bar of the per-link form's own FP32 tests); the
 plaquette form must not be further from the oracle than the per-link form
 is.  Shapes cover one block, several blocks along y and z (the halo row and
 the z slab boundaries, periodic wrap included), a z extent that the slab
-length does not divide, nx = 128 and 192, and anisotropic spacings."""
+length does not divide, nx = 64 to 384 (2 to 8 rows per block), and
+anisotropic spacings."""
 import numpy as np
 import pytest
 
@@ -18,7 +19,9 @@ import topologicalcymsolver_amd as T
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(64, 8, 8), (64, 16, 24), (64, 7, 6), (128, 4, 5), (192, 8, 13)]
+# PY (rows per block) 8, 8, 7, 4, 4, 2, 3, 2: with PY = 2 the second row builds
+# a halo plaquette and is also the last row
+SHAPES = [(64, 8, 8), (64, 16, 24), (64, 7, 6), (128, 4, 5), (192, 8, 13), (256, 4, 6), (256, 6, 4), (384, 4, 3)]
 SPACINGS = [((1.0, 1.0, 1.0), 1.0), ((1.0, 1.3, 0.7), 0.9)]
 KERNEL = {"leapfrog": "leapfrog", "langevin": "langevin", "flow": "flow"}
 
