@@ -69,6 +69,26 @@ def measured_traffic(nc, N, kind, fp32):
     return None, None
 
 
+def path_roofline(nc, V, r):
+    """The update kernels of the timed job together (every leapfrog and flow
+    launch): algorithmic bytes over their summed HIP-event time, and over the
+    job's wall time (measurements included)."""
+    tot_b, tot_s = 0.0, 0.0
+    for kind, n in (("leapfrog", r["n_lf"]), ("flow", r["n_flow"])):
+        ms, launches = r["stats"][kind]
+        tot_b += 3 * V * bytes_per_link_update(nc, kind, True) * launches
+        tot_s += ms / 1e3
+    if tot_s <= 0:
+        return None
+    job_b = 3 * V * (bytes_per_link_update(nc, "leapfrog", True) * r["n_lf"]
+                     + bytes_per_link_update(nc, "flow", True) * r["n_flow"])
+    return {"achieved": tot_b / tot_s / 1e9, "frac": tot_b / tot_s / 1e9 / HBM_PEAK_GBS, "unit": "GB/s",
+            "kernel_time_s": tot_s, "job_achieved": job_b / r["wall"] / 1e9,
+            "job_frac": job_b / r["wall"] / 1e9 / HBM_PEAK_GBS,
+            "basis": "algorithmic bytes of every leapfrog and flow launch over their summed kernel time "
+                     "(job_*: over the wall time of the timed job, measurements included)"}
+
+
 def make_initial_condition(T, nc, N, seed, su, se, maxiter):
     g = T.Solver(nc, N, T.FP64)
     g.random_links(seed, su)
@@ -481,6 +501,7 @@ def main():
                            "traffic is lower" % bpl,
         },
         "kernels": kernels,
+        "roofline_update_path": path_roofline(nc, V, r32),
         "fp32": {"wall_s": r32["wall"], "link_updates_per_s": lu32, "unitarity_max": r32["unitarity"],
                  "gauss_signed_max": r32["gauss"][0], "gauss_rms": r32["gauss"][1],
                  "delta_ncs": r32["ncs"][-1] if r32["ncs"] else None},
