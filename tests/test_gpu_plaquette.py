@@ -175,3 +175,27 @@ def test_plaquette_form_langevin_with_noise_field(nc):
     ref.langevin_xi(ref.noise(5, 11, sd), 0.05, 0.5)
     np.testing.assert_allclose(res[T.FORM_PLAQUETTE][0], ref.U, atol=5e-5)
     np.testing.assert_allclose(res[T.FORM_PLAQUETTE][1], ref.E, atol=5e-5)
+
+
+@pytest.mark.parametrize("N,pz", [((192, 192, 96), 8), ((192, 192, 72), 6)])
+def test_auto_form_with_shortened_slabs(N, pz):
+    """On flatter lattices TCYM_FORM_AUTO shortens the z slab (12 -> 8 or 6
+    planes) to keep two blocks per compute unit: the flow and leapfrog steps
+    there agree with the per-link form to FP32 rounding."""
+    print("auto form, slab of %d planes: SU(3) %s" % (pz, N), flush=True)
+    res = {}
+    for form in (T.FORM_AUTO, T.FORM_LINK):
+        g = T.Solver(3, N, T.FP32)
+        g.set_update_form(form)
+        g.random_links(17, 0.3)
+        g.random_efield(17, 0.3)
+        g.timing(True)
+        g.dynamics_update(2, 0.05)
+        g.gradient_flow(3, 0.02, u=T.U_HOT, e=T.E_HOT)
+        assert g.kernel_time("flow_plaq")[1] == (3 if form == T.FORM_AUTO else 0)
+        assert g.kernel_time("leapfrog_plaq")[1] == (2 if form == T.FORM_AUTO else 0)
+        res[form] = (g.get_links(), g.get_efield(), g.energy())
+        g.close()
+    assert np.abs(res[T.FORM_AUTO][0] - res[T.FORM_LINK][0]).max() < 5e-5
+    assert np.abs(res[T.FORM_AUTO][1] - res[T.FORM_LINK][1]).max() < 5e-5
+    assert abs(res[T.FORM_AUTO][2] - res[T.FORM_LINK][2]) < 1e-5 * abs(res[T.FORM_LINK][2])
