@@ -290,25 +290,25 @@ def cpu_baseline(nc, N, U, E, dt, dtau, budget_s):
                                          % (one["n_lf"], one["n_fl"], one["wall"])})
 
 
-SMALL_CONFIGS = {
-    1: dict(nc=2, L=16, seed=1001, su=0.2, se=0.3, forms=("fp64",), phases=[("leapfrog", 200, 0.05)],
+SMALL_WORKLOADS = {
+    "su2-16": dict(nc=2, L=16, seed=1001, su=0.2, se=0.3, forms=("fp64",), phases=[("leapfrog", 200, 0.05)],
             text="SU(2) 16^3 FP64, seeded links (sigma 0.2) and E (sigma 0.3) Gauss-projected; 200 leapfrog steps "
                  "dt=0.05 with the hot N_CS integral (E.B at every step) on the device"),
-    2: dict(nc=2, L=64, seed=2002, su=0.2, se=0.3, forms=("fp32", "fp64"),
+    "su2-64": dict(nc=2, L=64, seed=2002, su=0.2, se=0.3, forms=("fp32", "fp64"),
             phases=[("leapfrog", 1000, 0.05), ("langevin", 1000, 0.05)],
-            text="SU(2) 64^3 FP64 vs FP32-mixed, seeded IC as config 1; 1000 leapfrog steps dt=0.05 then 1000 Langevin "
+            text="SU(2) 64^3 FP64 vs FP32-mixed, seeded IC as the SU(2) 16^3 job; 1000 leapfrog steps dt=0.05 then 1000 Langevin "
                  "steps (damping 0.1, T 0.5, Philox noise), hot N_CS integral at every step, energy every 100 steps"),
-    3: dict(nc=3, L=96, seed=3003, su=0.15, se=0.25, forms=("fp32", "fp64"),
+    "su3-96": dict(nc=3, L=96, seed=3003, su=0.15, se=0.25, forms=("fp32", "fp64"),
             phases=[("leapfrog", 500, 0.04), ("flow", 500, 0.02)],
             text="SU(3) 96^3 FP64 vs FP32-mixed, links sigma 0.15, E sigma 0.25 Gauss-projected; 500 leapfrog steps "
                  "dt=0.04 (hot N_CS integral) then 500 gradient-flow steps dtau=0.02 on a copy"),
 }
 
 
-def run_small(T, cfg, prec, U, E):
+def run_small(T, name, prec, U, E):
     """One form of a small configuration: the phases in calls of 100 steps,
     the energy after each call; the wall time of the steps only."""
-    c = SMALL_CONFIGS[cfg]
+    c = SMALL_WORKLOADS[name]
     N = (c["L"],) * 3
     g = T.Solver(c["nc"], N, prec)
     g.set_links(U)
@@ -344,17 +344,17 @@ def run_small(T, cfg, prec, U, E):
     return out
 
 
-def run_config(T, cfg, cpu_seconds):
-    """Configurations 1-3 (the parity-test cases of the path, timed): one JSON
-    line with the same fields as the default workload's."""
-    c = SMALL_CONFIGS[cfg]
+def run_workload(T, name, cpu_seconds):
+    """The smaller jobs of the path (SU(2) 16^3 and 64^3, SU(3) 96^3), timed:
+    one JSON line with the same fields as the default workload's."""
+    c = SMALL_WORKLOADS[name]
     N = (c["L"],) * 3
-    log("# config %d: %s" % (cfg, c["text"]))
+    log("# %s: %s" % (name, c["text"]))
     U, E, it, fm = make_initial_condition(T, c["nc"], N, c["seed"], c["su"], c["se"], 10000)
     res = {}
     for f in c["forms"]:
-        run_small(T, cfg, T.FP32 if f == "fp32" else T.FP64, U, E)      # warm-up run
-        res[f] = run_small(T, cfg, T.FP32 if f == "fp32" else T.FP64, U, E)
+        run_small(T, name, T.FP32 if f == "fp32" else T.FP64, U, E)      # warm-up run
+        res[f] = run_small(T, name, T.FP32 if f == "fp32" else T.FP64, U, E)
         log("# %s: %.3f s, %.3e link-updates/s" % (f, res[f]["wall"], res[f]["lu"]))
     main_form = "fp32" if "fp32" in res else "fp64"
     r = res[main_form]
@@ -364,7 +364,7 @@ def run_config(T, cfg, cpu_seconds):
         "ms_per_step": 1e3 * r["wall"] / r["steps"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32" if main_form == "fp32" else "f64",
         "data": "synthetic (seeded Philox initial condition, Gauss-projected)",
-        "config": {"workload": "config %d: %s" % (cfg, c["text"]), "gauss_restore_iterations": it,
+        "config": {"workload": c["text"], "gauss_restore_iterations": it,
                    "gauss_restore_signed_max": fm},
     }
     for f, x in res.items():
@@ -380,12 +380,12 @@ def run_config(T, cfg, cpu_seconds):
         import _oracle as O
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         smp = cpu_sample(O, c["nc"], N, U, E, c["phases"][0][2], 0.02, cpu_seconds, threads)
-        rate = smp["lf"] if cfg != 3 else 1000.0 / (500.0 / smp["lf"] + 500.0 / smp["fl"])
+        rate = smp["lf"] if name != "su3-96" else 1000.0 / (500.0 / smp["lf"] + 500.0 / smp["fl"])
         out["cpu_baseline"] = {"value": rate, "unit": "link-updates/s", "cores": threads, "kind": "port",
                                "cpu": cpu_model(),
                                "sample": "%d leapfrog + %d flow steps of the oracle (FP64, OpenMP, %d threads, %.1f s)%s"
                                          % (smp["n_lf"], smp["n_fl"], threads, smp["wall"],
-                                            ", rate weighted 500 : 500" if cfg == 3 else ", leapfrog rate")}
+                                            ", rate weighted 500 : 500" if name == "su3-96" else ", leapfrog rate")}
     print(json.dumps(out), flush=True)
 
 
@@ -409,9 +409,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensembles workload")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=500)
-    ap.add_argument("--config", type=int, default=5, choices=(1, 2, 3, 5),
-                    help="5: the SU(3) 192^3 job (default; with the batched workload, config 4); 1-3: the smaller "
-                         "configurations of the path, timed alone")
+    ap.add_argument("--workload", default="su3-192", choices=("su3-192", "su2-16", "su2-64", "su3-96"),
+                    help="su3-192: the SU(3) 192^3 leapfrog + cooling job (default; with the batched SU(2) "
+                         "ensembles); the others: the smaller jobs of the path, timed alone")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -420,8 +420,8 @@ def main():
         sys.exit(2)
 
     import topologicalcymsolver_amd as T
-    if args.config != 5:
-        run_config(T, args.config, args.cpu_seconds)
+    if args.workload != "su3-192":
+        run_workload(T, args.workload, args.cpu_seconds)
         return
     nc, N = args.nc, (args.N, args.N, args.N)
     V = N[0] * N[1] * N[2]
