@@ -599,22 +599,27 @@ struct Log3 {
     __device__ static __forceinline__ C cj(C a) { return {a.re, -a.im}; }
     __device__ static __forceinline__ C csc(double s, C a) { return {s*a.re, s*a.im}; }
 
-    __device__ static void mm(const C* A, const C* B, C* R, bool dA) {
+    __device__ static __forceinline__ void mm(const C* A, const C* B, C* R) {
         C T[9];
-        for (int r = 0; r < 3; r++) for (int c = 0; c < 3; c++) {
-            C s = {0, 0};
-            for (int k = 0; k < 3; k++) s = ca(s, cm(dA ? cj(A[k + 3*r]) : A[r + 3*k], B[k + 3*c]));
-            T[r + 3*c] = s;
-        }
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                C s = {0, 0};
+#pragma unroll
+                for (int k = 0; k < 3; k++) s = ca(s, cm(A[r + 3*k], B[k + 3*c]));
+                T[r + 3*c] = s;
+            }
+#pragma unroll
         for (int k = 0; k < 9; k++) R[k] = T[k];
     }
-    __device__ static C det3(const C* M) {
+    __device__ static __forceinline__ C det3(const C* M) {
         C c00 = cs(cm(M[4], M[8]), cm(M[7], M[5]));
         C c01 = cs(cm(M[7], M[2]), cm(M[1], M[8]));
         C c02 = cs(cm(M[1], M[5]), cm(M[4], M[2]));
         return ca(ca(cm(M[0], c00), cm(M[3], c01)), cm(M[6], c02));
     }
-    __device__ static void inv3(const C* M, C* I) {
+    __device__ static __forceinline__ void inv3(const C* M, C* I) {
         C c00 = cs(cm(M[4], M[8]), cm(M[7], M[5]));
         C c01 = cs(cm(M[7], M[2]), cm(M[1], M[8]));
         C c02 = cs(cm(M[1], M[5]), cm(M[4], M[2]));
@@ -631,62 +636,80 @@ struct Log3 {
         I[1] = cm(c01, id); I[4] = cm(c11, id); I[7] = cm(c21, id);
         I[2] = cm(c02, id); I[5] = cm(c12, id); I[8] = cm(c22, id);
     }
+    // one Jacobi rotation in the (p, q) plane: A <- R^+ A R, V <- V R, with R
+    // the identity but for R_pp = c, R_pq = s, R_qp = -s conj(e),
+    // R_qq = c conj(e), e = A_pq/|A_pq|; only columns p, q of A R and V R and
+    // rows p, q of R^+ (A R) change
+    template <int p, int q>
+    __device__ static __forceinline__ void rot(C* A, C* V) {
+        const C apq = A[p + 3*q];
+        const double g = __builtin_sqrt(apq.re*apq.re + apq.im*apq.im);
+        if (g == 0.0) return;
+        const C e = csc(1.0/g, apq), ec = cj(e);
+        const double th = (A[q + 3*q].re - A[p + 3*p].re)/(2.0*g);
+        const double t = (th >= 0 ? 1.0 : -1.0)/(fabs(th) + __builtin_sqrt(th*th + 1.0));
+        const double cc = 1.0/__builtin_sqrt(t*t + 1.0), sn = t*cc;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const C u = cm(A[r + 3*q], ec), a = A[r + 3*p];
+            A[r + 3*p] = cs(csc(cc, a), csc(sn, u));
+            A[r + 3*q] = ca(csc(sn, a), csc(cc, u));
+            const C v = cm(V[r + 3*q], ec), b = V[r + 3*p];
+            V[r + 3*p] = cs(csc(cc, b), csc(sn, v));
+            V[r + 3*q] = ca(csc(sn, b), csc(cc, v));
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const C u = cm(e, A[q + 3*c]), a = A[p + 3*c];
+            A[p + 3*c] = cs(csc(cc, a), csc(sn, u));
+            A[q + 3*c] = ca(csc(sn, a), csc(cc, u));
+        }
+        A[p + 3*q] = A[q + 3*p] = {0, 0};
+#pragma unroll
+        for (int k = 0; k < 3; k++) A[k + 3*k].im = 0.0;
+    }
     // hermitian Jacobi; A overwritten, V eigenvectors (columns), w eigenvalues
-    __device__ static void jacobi(C* A, C* V, double* w) {
+    __device__ static __forceinline__ void jacobi(C* A, C* V, double* w) {
+#pragma unroll
         for (int k = 0; k < 9; k++) V[k] = {0, 0};
         V[0] = V[4] = V[8] = {1, 0};
         for (int sweep = 0; sweep < 12; sweep++) {
             double off = A[3].re*A[3].re + A[3].im*A[3].im + A[6].re*A[6].re + A[6].im*A[6].im + A[7].re*A[7].re + A[7].im*A[7].im;
             double dia = A[0].re*A[0].re + A[4].re*A[4].re + A[8].re*A[8].re;
             if (off == 0.0 || off < 1e-36*dia) break;
-            for (int pq = 0; pq < 3; pq++) {
-                const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
-                C apq = A[p + 3*q];
-                double g = __builtin_sqrt(apq.re*apq.re + apq.im*apq.im);
-                if (g == 0.0) continue;
-                C e = csc(1.0/g, apq);
-                double th = (A[q + 3*q].re - A[p + 3*p].re)/(2.0*g);
-                double t = (th >= 0 ? 1.0 : -1.0)/(fabs(th) + __builtin_sqrt(th*th + 1.0));
-                double cc = 1.0/__builtin_sqrt(t*t + 1.0), sn = t*cc;
-                C R[9];
-                for (int k = 0; k < 9; k++) R[k] = {0, 0};
-                for (int k = 0; k < 3; k++) R[k + 3*k] = {1, 0};
-                R[p + 3*p] = {cc, 0};
-                R[p + 3*q] = {sn, 0};
-                R[q + 3*p] = csc(-sn, cj(e));
-                R[q + 3*q] = csc(cc, cj(e));
-                C T[9];
-                mm(A, R, T, false);
-                mm(R, T, A, true);
-                mm(V, R, T, false);
-                for (int k = 0; k < 9; k++) V[k] = T[k];
-                A[p + 3*q] = A[q + 3*p] = {0, 0};
-                for (int k = 0; k < 3; k++) A[k + 3*k].im = 0.0;
-            }
+            rot<0, 1>(A, V);
+            rot<0, 2>(A, V);
+            rot<1, 2>(A, V);
         }
+#pragma unroll
         for (int k = 0; k < 3; k++) w[k] = A[k + 3*k].re;
     }
     // W given as re/im arrays (row + 3 col); out al with exp(i c al t) = W
     __device__ static void logi(double c, const double* Wr, const double* Wi, double* al) {
         C W[9];
         for (int k = 0; k < 9; k++) W[k] = {Wr[k], Wi[k]};
+        // shifts 0, pi/2, -pi/2, pi and their phases e^{-i shift}
         const double shifts[4] = {0.0, 1.5707963267948966, -1.5707963267948966, 3.141592653589793};
+        const C phases[4] = {{1.0, 0.0}, {0.0, -1.0}, {0.0, 1.0}, {-1.0, 0.0}};
         double best = -1.0, sft = 0.0;
+        C ph = phases[0];
+#pragma unroll
         for (int k = 0; k < 4; k++) {
-            C ph = {cos(shifts[k]), -sin(shifts[k])}, M[9];
-            for (int j = 0; j < 9; j++) M[j] = cm(ph, W[j]);
+            C M[9];
+#pragma unroll
+            for (int j = 0; j < 9; j++) M[j] = cm(phases[k], W[j]);
             M[0].re += 1; M[4].re += 1; M[8].re += 1;
             C d = det3(M);
             double ad = d.re*d.re + d.im*d.im;
-            if (ad > best) { best = ad; sft = shifts[k]; }
+            if (ad > best) { best = ad; sft = shifts[k]; ph = phases[k]; }
             if (k == 0 && ad > 0.25) break;
         }
-        C ph = {cos(sft), -sin(sft)}, P[9], Mi[9];
+        C P[9], Mi[9];
         for (int j = 0; j < 9; j++) { C wp = cm(ph, W[j]); P[j] = wp; Mi[j] = csc(-1.0, wp); }
         for (int j = 0; j < 3; j++) { P[j + 3*j].re += 1; Mi[j + 3*j].re += 1; }
         C Pinv[9], Cm[9], Ch[9];
         inv3(P, Pinv);
-        mm(Mi, Pinv, Cm, false);
+        mm(Mi, Pinv, Cm);
         for (int r = 0; r < 3; r++) for (int cc = 0; cc < 3; cc++) {
             C a = cm({0, 1}, Cm[r + 3*cc]), b = cj(cm({0, 1}, Cm[cc + 3*r]));
             Ch[r + 3*cc] = csc(0.5, ca(a, b));
