@@ -158,6 +158,12 @@ void or_thermal_efield(const or_params *p, uint64_t seed, uint32_t cycle, double
 void or_leapfrog(const or_params *p, double *U, double *E, int n, double dt) {
     DISPATCH(p, su2l_leapfrog(p, U, E, n, dt), su3l_leapfrog(p, U, E, n, dt));
 }
+void or_update_efield(const or_params *p, const double *U, double *E, double dt) {
+    DISPATCH(p, su2l_update_efield(p, U, E, dt), su3l_update_efield(p, U, E, dt));
+}
+void or_update_links(const or_params *p, double *U, const double *E, double dt) {
+    DISPATCH(p, su2l_update_links(p, U, E, dt), su3l_update_links(p, U, E, dt));
+}
 void or_langevin_xi(const or_params *p, double *U, double *E, const double *Xi, double dt, double damping) {
     DISPATCH(p, su2l_langevin_xi(p, U, E, Xi, dt, damping), su3l_langevin_xi(p, U, E, Xi, dt, damping));
 }

@@ -64,6 +64,8 @@ int  or_thermalize(const or_params *p, double *U, double *E, int cycles, double 
 
 /* ---- lattice sweeps ---- */
 void or_leapfrog(const or_params *p, double *U, double *E, int nsteps, double dt);
+void or_update_efield(const or_params *p, const double *U, double *E, double dt);   /* E += gamma F */
+void or_update_links(const or_params *p, double *U, const double *E, double dt);   /* U <- exp(i cU E) U */
 void or_langevin_xi(const or_params *p, double *U, double *E, const double *Xi,
                     double dt, double damping);
 void or_langevin(const or_params *p, double *U, double *E, int nsteps, double dt,

@@ -52,6 +52,7 @@ def lib():
             "or_random_links": (None, [P, u64, d, _dp]), "or_random_efield": (None, [P, u64, d, _dp]),
             "or_noise_field": (None, [P, u64, u32, d, _dp]),
             "or_leapfrog": (None, [P, _dp, _dp, i, d]),
+            "or_update_efield": (None, [P, _dp, _dp, d]), "or_update_links": (None, [P, _dp, _dp, d]),
             "or_langevin_xi": (None, [P, _dp, _dp, _dp, d, d]),
             "or_langevin": (None, [P, _dp, _dp, i, d, d, d, u64, u32]),
             "or_flow_force": (None, [P, _dp, _dp]), "or_flow_links": (None, [P, _dp, _dp, d]),
@@ -125,6 +126,14 @@ class Lattice:
     # ---- sweeps ----
     def leapfrog(self, n, dt):
         lib().or_leapfrog(ctypes.byref(self.p), _ptr(self.U), _ptr(self.E), n, dt)
+
+    def update_efield(self, dt):
+        """Dynamics::UpdateElectricFields alone."""
+        lib().or_update_efield(ctypes.byref(self.p), _ptr(self.U), _ptr(self.E), dt)
+
+    def update_links(self, dt):
+        """Dynamics::UpdateGaugeLinks alone."""
+        lib().or_update_links(ctypes.byref(self.p), _ptr(self.U), _ptr(self.E), dt)
 
     def langevin(self, n, dt, damping, T, seed, step0=0):
         lib().or_langevin(ctypes.byref(self.p), _ptr(self.U), _ptr(self.E), n, dt, damping, T, seed, step0)
