@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TFLOPS = 78.6
 
 
 def log(*a):
@@ -66,6 +67,26 @@ def measured_traffic(nc, N, kind, fp32):
             continue
         if (t.get("kernel"), t.get("dtype"), t.get("nc"), t.get("N")) == (kind, "f32" if fp32 else "f64", nc, N):
             return t["traffic_bytes"], "profiles/" + t.get("source", f)
+    return None, None
+
+
+def measured_flops(nc, N, kind, form):
+    """FP32 and FP64 VALU flops per launch of the dominant kernel, from the
+    SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F{32,64} counters of a separate profiler
+    run whose summary is kept in profiles/; None if none matches."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None, None
+    for f in sorted(os.listdir(d)):
+        if not f.endswith("_valu.json"):
+            continue
+        try:
+            with open(os.path.join(d, f)) as fh:
+                t = json.load(fh)
+        except (OSError, ValueError):
+            continue
+        if (t.get("kernel"), t.get("dtype"), t.get("nc"), t.get("N"), t.get("form")) == (kind, "f32", nc, N, form):
+            return t["flop_f32"] + t["flop_f64"], "profiles/" + t.get("source", f)
     return None, None
 
 
@@ -454,6 +475,7 @@ def main():
     bpl = bytes_per_link_update(nc, dom, True)
     achieved = 3 * V * bpl / avg_s / 1e9
     traffic, traffic_src = measured_traffic(nc, args.N, dom, True)
+    flops, flops_src = measured_flops(nc, args.N, dom, r32["forms"].get(dom))
     kernels = {}
     for k, (ms, n) in r32["stats"].items():
         if n:
@@ -499,6 +521,12 @@ def main():
             "bytes_basis": "algorithmic: every link read once and written once per step (2 s_U w = %d B); between the "
                            "steps of one call the kernel moves columns 0 and 1 of the links only, so the measured "
                            "traffic is lower" % bpl,
+            "valu": None if flops is None else {
+                "achieved": flops / avg_s / 1e12, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": flops / avg_s / 1e12 / VALU_PEAK_TFLOPS, "flop_per_launch": flops, "source": flops_src,
+                "basis": "FP32 + FP64 VALU flops per launch (SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F{32,64} counters, "
+                         "FMA = 2) over the live average launch time; peak = non-packed vector math, 256 CUs x 4 "
+                         "SIMDs x 16 lanes x 2 x 2.4 GHz (packed FP32 is disabled in this build)"},
         },
         "kernels": kernels,
         "roofline_update_path": path_roofline(nc, V, r32),
