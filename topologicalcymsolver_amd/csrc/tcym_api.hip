@@ -110,6 +110,7 @@ struct tcym_ctx : tcym::DeviceQueue {
     double* U64 = nullptr;
     double* E64 = nullptr;
     double* map = nullptr;
+    void* Bc = nullptr;          // field strength of one link slot (tcym_evolve_ncs, within a call)
 
     // Resolution level of each slot: 0 is the full lattice; level l is the
     // lattice blocked l times (extent N >> l, spacing a << l), which a
@@ -443,13 +444,17 @@ int tcym_evolve_ncs(tcym_ctx* c, int mode, int u, int e, int nsteps, double dt, 
     const double zero = 0.0;
     if ((r = c->hip(hipMemcpyAsync(acc, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync(acc)"))) return r;
     const double half_dt = 0.5*dt;
+    // E.B is linear in E: the field strength B of U_n+1, formed for the
+    // second E.B of a step, is kept and serves the first E.B of the next
+    // step (on the same links), so each step forms B once
+    if (nsteps > 1 && (r = c->alloc(&c->Bc, (size_t)c->field_elems()*c->be->elsize, "hipMalloc(field strength)"))) return r;
     for (int k = 0; k < nsteps; k++) {
         c->be->update(c->L(), q, mode, true, true, dt, damping, noise, seed, step0 + (uint32_t)k, c->U[u], c->Uscr, c->E[e], nullptr,
                       true);
         std::swap(c->U[u], c->Uscr);     // Uscr now holds U_n
-        long nb = c->be->edotb(c->L(), q, 1, c->Uscr, c->E[e], c->part, nullptr);
+        long nb = c->be->edotb_cached(c->L(), q, c->Uscr, c->E[e], c->part, c->Bc, k == 0 ? 0 : 2);
         tcym::final_reduce(c->L(), c->part, nb, 1, &mx, eb);
-        nb = c->be->edotb(c->L(), q, 1, c->U[u], c->E[e], c->part, nullptr);
+        nb = c->be->edotb_cached(c->L(), q, c->U[u], c->E[e], c->part, c->Bc, k + 1 < nsteps ? 1 : 0);
         tcym::final_reduce(c->L(), c->part, nb, 1, &mx, eb + 1);
         c->launch("hot_ncs", [&](hipStream_t st) { hipLaunchKernelGGL(k_hot_ncs, dim3(1), dim3(64), 0, st, (const double*)eb, acc, half_dt); });
         if (c->last != hipSuccess) return TCYM_E_HIP;

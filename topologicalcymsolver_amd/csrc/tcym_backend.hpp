@@ -103,6 +103,9 @@ struct Backend {
     // reductions: per-block partials (double) written to `part`; return number of blocks
     virtual long edotb(const Launcher&, const Geom&, int improved, const void* U, const void* E, double* part, double* map) = 0;
     // the improved E.B partials launched straight onto a stream (no launcher object, no allocation)
+    // improved E.B with the field strength B of (U) formed (bmode 0), formed
+    // and kept in Bc (1), or taken from Bc (2); block partials as edotb
+    virtual long edotb_cached(const Launcher&, const Geom&, const void* U, const void* E, double* part, void* Bc, int bmode) = 0;
     virtual long edotb_on(hipStream_t, const Geom&, const void* U, const void* E, double* part) = 0;
     virtual long bulk(const Launcher&, const Geom&, const void* U, const void* E, double* part) = 0;
     virtual long gauss(const Launcher&, const Geom&, const void* U, const void* E, double* part, double* G) = 0;
