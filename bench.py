@@ -239,7 +239,8 @@ def run_batched(T, count, seed, precision, lang_steps, seg, cool_every, cool_ste
     b.sync()
     wall = time.perf_counter() - t0
     kern = {}
-    for name in ("batch_langevin", "batch_flow", "batch_edotb", "batch_geodesic", "batch_reduce", "copy"):
+    for name in ("batch_langevin", "batch_langevin_plaq", "batch_flow", "batch_flow_plaq", "batch_edotb",
+                 "batch_geodesic", "batch_reduce", "copy"):
         ms, n = b.kernel_time(name)
         if n:
             kern[name] = {"launches": n, "avg_us": 1e3 * ms / n}
@@ -561,7 +562,8 @@ def main():
         b64 = run_batched(T, args.batch_count, 4004, T.FP64, tune=True, **bkw)
         log("# batched FP64: %.3f s, %.3e link-updates/s" % (b64["wall"], b64["lu"]))
         kd = max(b32["kernels"], key=lambda k: b32["kernels"][k]["avg_us"] * b32["kernels"][k]["launches"])
-        kind = {"batch_langevin": "langevin", "batch_flow": "flow"}.get(kd)
+        kind = {"batch_langevin": "langevin", "batch_flow": "flow", "batch_langevin_plaq": "langevin",
+                "batch_flow_plaq": "flow"}.get(kd)
         roof = None
         if kind:
             bpl = bytes_per_link_update(2, kind, True)

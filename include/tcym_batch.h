@@ -95,17 +95,29 @@ int tcym_batch_bulk(tcym_batch *b, int u, int e, double *out);
 int tcym_batch_gauss(tcym_batch *b, int u, int e, double *out);
 int tcym_batch_unitarity(tcym_batch *b, int u, double *out);
 
-/* Autotuner of the update kernels: times every candidate walk shape
- * (canonical, linear, 64/32/16-wide patches) and persistent grid size on
+/* Autotuner of the update kernels: times every candidate shape (per-link
+ * form: canonical, linear and 64/32/16-wide patch walks, shapes -1, 0, 4..6;
+ * FP32 batches whose rows fit a 256-thread block also the plaquette form with
+ * units of 4, 8 or 16 planes, shapes 7..9) and persistent grid size on
  * scratch copies of the fields and keeps the fastest for one pass: mode 0 or
  * 1 tunes the leapfrog and Langevin updates (repeated launches on one
  * input), mode 3 the flow (sequences of flow steps alternating between two
  * link buffers with two-column stores, as in a cooling episode); each pass
  * keeps its own choice.  The timing runs overwrite the free slots
- * TCYM_U_AUX and TCYM_E_AUX.  Results do not depend on the choice.  choice
- * receives {shape, blocks per CU, XCD split} and us the best time per
- * update step in microseconds (both may be NULL). */
+ * TCYM_U_AUX and TCYM_E_AUX.  Within a form results do not depend on the
+ * choice (each lattice's fields equal those of a tcym_ctx run in that form,
+ * TCYM_FORM_LINK or TCYM_FORM_PLAQUETTE); the two forms agree to FP32
+ * rounding.  choice receives {shape, blocks per CU, XCD split} and us the
+ * best time per update step in microseconds (both may be NULL). */
 int tcym_batch_tune(tcym_batch *b, int mode, double dt, int reps, int choice[3], double *us);
+
+/* The schedule of one pass set by hand (mode 0 or 1: leapfrog and Langevin;
+ * mode 3: flow): shape as in tcym_batch_tune, blocks per compute unit of the
+ * persistent grid (1..64), XCD ranges on or off.  No reference counterpart
+ * (the reference runs one lattice per process); for tests and for callers
+ * that keep a tuned schedule across runs.  TCYM_E_UNSUPPORTED for a
+ * plaquette shape on an FP64 batch or on rows that do not fit its blocks. */
+int tcym_batch_set_schedule(tcym_batch *b, int mode, int shape, int blocks_per_cu, int xcd_split);
 
 /* Kernel timing as tcym_timing / tcym_kernel_time / tcym_timing_reset. */
 int tcym_batch_timing(tcym_batch *b, int enable);

@@ -154,9 +154,12 @@ def test_odd_segment_length_alternates_buffers():
 def test_autotuned_batch_is_bit_identical(nc, sizes):
     b, singles = build(nc, sizes, T.FP32)
     choice, us = b.tune(mode=1, dt=0.05, reps=2)
-    assert choice[0] in (-1, 0, 4, 5, 6) and choice[1] in (2, 4, 8, 16) and us > 0
+    assert choice[0] in (-1, 0, 4, 5, 6, 7, 8, 9) and choice[1] in (2, 4, 8, 16) and us > 0
     # the tuner ran on copies: the fields are untouched
     assert_same_fields(b, singles)
+    # the single lattices run the form the tuner picked
+    for s in singles:
+        s.set_update_form(T.FORM_PLAQUETTE if choice[0] >= 7 else T.FORM_LINK)
     b.langevin(3, 0.05, 0.1, 0.5, NOISE0)
     b.leapfrog(2, 0.05)
     for e, s in enumerate(singles):
@@ -173,12 +176,71 @@ def test_autotuned_batch_is_bit_identical(nc, sizes):
     assert_same_fields(b, singles)
     # the flow pass keeps a choice of its own
     fchoice, fus = b.tune(mode=3, dt=0.02, reps=2)
-    assert fchoice[0] in (-1, 0, 4, 5, 6) and fchoice[1] in (2, 4, 8, 16) and fus > 0
+    assert fchoice[0] in (-1, 0, 4, 5, 6, 7, 8, 9) and fchoice[1] in (2, 4, 8, 16) and fus > 0
     assert_same_fields(b, singles)
+    for s in singles:
+        s.set_update_form(T.FORM_PLAQUETTE if fchoice[0] >= 7 else T.FORM_LINK)
     b.flow(5, 0.02, u=T.U_HOT)
     for s in singles:
         s.gradient_flow(5, 0.02, u=T.U_HOT, e=-1)
     assert_same_fields(b, singles)
+
+
+@pytest.mark.parametrize("nc,sizes", [(2, SU2_SIZES + [(16, 24, 12), (48, 48, 20), (40, 10, 6)]), (3, SU3_SIZES)])
+@pytest.mark.parametrize("shape", [7, 8, 9])
+def test_plaquette_batch_matches_single_lattices(nc, sizes, shape):
+    """The plaquette form in a batch (256-thread blocks: a whole plane per
+    block where it fits, else bands of a few rows with a recomputed halo row
+    and idle threads beyond them; units of 4, 8 or 16 planes) against each
+    lattice alone in the plaquette form (768-thread blocks, other bands and
+    slabs): every site's force is the same sum in the same order, so the
+    fields and measurements are the same to the bit, through Langevin,
+    leapfrog, flow and graph-replayed segments."""
+    b, singles = build(nc, sizes, T.FP32)
+    for s in singles:
+        s.set_update_form(T.FORM_PLAQUETTE)
+    b.set_schedule(1, shape, 4, True)
+    b.set_schedule(3, shape, 4, True)
+    b.timing(True)
+    b.langevin(3, 0.05, 0.1, 0.5, NOISE0)
+    b.leapfrog(2, 0.05)
+    for e, s in enumerate(singles):
+        s.langevin_update(3, 0.05, 0.1, 0.5, NOISE0 + e, 0)
+        s.dynamics_update(2, 0.05)
+    assert_same_fields(b, singles)
+    assert_same_measurements(b, singles)
+    traj = b.evolve(1, 2, 2, 0.05, 0.1, 0.5, NOISE0)
+    for k in range(2):
+        for e, s in enumerate(singles):
+            s.langevin_update(2, 0.05, 0.1, 0.5, NOISE0 + e, 3 + 2 * k)
+            assert traj[e, k] == s.ncs_dot(), (e, k)
+    assert_same_fields(b, singles)
+    b.flow(5, 0.02, u=T.U_HOT)
+    for s in singles:
+        s.gradient_flow(5, 0.02, u=T.U_HOT, e=-1)
+    assert_same_fields(b, singles, slot_e=None)
+    # the plaquette kernels ran, and only they
+    assert b.kernel_time("batch_langevin_plaq")[1] == 3 and b.kernel_time("batch_leapfrog_plaq")[1] == 2
+    assert b.kernel_time("batch_flow_plaq")[1] == 5 and b.kernel_time("batch_flow")[1] == 0
+    assert b.kernel_time("batch_langevin")[1] == 0
+    b.close()
+    for s in singles:
+        s.close()
+
+
+def test_plaquette_batch_schedule_checked():
+    b = T.Batch(2, [(4, 4, 4), (6, 6, 6)], T.FP64)
+    with pytest.raises(T.TcymError):
+        b.set_schedule(3, 7)            # FP64 storage: per-link form only
+    b.close()
+    b = T.Batch(2, [(4, 4, 4), (200, 7, 4)], T.FP32)
+    with pytest.raises(T.TcymError):
+        b.set_schedule(1, 8)            # rows of 200 sites do not fit a 256-thread block in bands
+    for bad in ((1, 10), (1, 3), (2, 7)):
+        with pytest.raises(T.TcymError):
+            b.set_schedule(*bad)
+    b.set_schedule(1, 0, 4, False)
+    b.close()
 
 
 def test_batch_rejects_bad_arguments():

@@ -1,6 +1,7 @@
 """The plaquette form of the update kernels (k_update_plaq: each plaquette
-multiplied out once and shared among its four links, FP32 storage, lattices
-whose x extent is a multiple of 64) against the oracle and against the
+multiplied out once and shared among its four links, FP32 storage; by
+default on lattices whose x extent is a multiple of 64, forced on any whose
+rows fit a block) against the oracle and against the
 per-link form (TCYM_FORM_LINK) on the same seeded inputs.
 
 Both forms sum the same force terms in FP32, in different orders, so they
@@ -20,8 +21,11 @@ import topologicalcymsolver_amd as T
 pytestmark = pytest.mark.gpu
 
 # PY (rows per block) 8, 8, 7, 4, 4, 2, 3, 2: with PY = 2 the second row builds
-# a halo plaquette and is also the last row
-SHAPES = [(64, 8, 8), (64, 16, 24), (64, 7, 6), (128, 4, 5), (192, 8, 13), (256, 4, 6), (256, 6, 4), (384, 4, 3)]
+# a halo plaquette and is also the last row.  Rows that are not whole waves
+# (forced plaquette form): (16, 16, 5) and (24, 6, 7) a whole plane per block,
+# (40, 30, 4) and (48, 18, 5) bands of 15 and 9 rows with a halo row
+SHAPES = [(64, 8, 8), (64, 16, 24), (64, 7, 6), (128, 4, 5), (192, 8, 13), (256, 4, 6), (256, 6, 4), (384, 4, 3),
+          (16, 16, 5), (24, 6, 7), (40, 30, 4), (48, 18, 5)]
 SPACINGS = [((1.0, 1.0, 1.0), 1.0), ((1.0, 1.3, 0.7), 0.9)]
 KERNEL = {"leapfrog": "leapfrog", "langevin": "langevin", "flow": "flow"}
 
@@ -126,7 +130,7 @@ def test_update_form_argument_checked():
         g.set_update_form(7)
     g.close()
     # shapes the plaquette form does not take run the per-link kernel
-    for N in [(48, 8, 8), (64, 13, 8), (448, 4, 4)]:
+    for N in [(400, 4, 4), (64, 13, 8), (448, 4, 4)]:
         g = T.Solver(3, N, T.FP32)
         g.set_update_form(T.FORM_PLAQUETTE)
         g.timing(True)

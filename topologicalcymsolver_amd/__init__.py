@@ -50,7 +50,7 @@ BATCH_EXPORTS = [
     "tcym_batch_get_efield", "tcym_batch_initial_condition", "tcym_batch_set_step", "tcym_batch_get_step",
     "tcym_batch_leapfrog", "tcym_batch_langevin", "tcym_batch_flow", "tcym_batch_evolve",
     "tcym_batch_cooling_start", "tcym_batch_cooling_update", "tcym_batch_ncsdot", "tcym_batch_bulk",
-    "tcym_batch_gauss", "tcym_batch_unitarity", "tcym_batch_tune", "tcym_batch_timing",
+    "tcym_batch_gauss", "tcym_batch_unitarity", "tcym_batch_tune", "tcym_batch_set_schedule", "tcym_batch_timing",
     "tcym_batch_kernel_time", "tcym_batch_timing_reset",
 ]
 
@@ -135,6 +135,7 @@ def load_library(path=LIB_PATH):
         "tcym_batch_ncsdot": [V, I, I, _dp], "tcym_batch_bulk": [V, I, I, _dp],
         "tcym_batch_gauss": [V, I, I, _dp], "tcym_batch_unitarity": [V, I, _dp],
         "tcym_batch_tune": [V, I, D, I, ctypes.POINTER(I), _dp], "tcym_batch_timing": [V, I],
+        "tcym_batch_set_schedule": [V, I, I, I, I],
         "tcym_batch_kernel_time": [V, ctypes.c_char_p, _dp, LL], "tcym_batch_timing_reset": [V],
         "tcym_get_params": [V, P],
         "tcym_thermalize": [V, I, I, I, D, D, D, U64, I, D, _dp, I, ctypes.POINTER(I)],
@@ -662,6 +663,12 @@ class Batch:
         us = ctypes.c_double()
         self._ck(self.L.tcym_batch_tune(self.h, mode, dt, reps, ch, ctypes.byref(us)))
         return tuple(ch), us.value
+
+    def set_schedule(self, mode, shape, blocks_per_cu=4, xcd_split=True):
+        """The schedule of one pass (mode 0/1: leapfrog and Langevin, 3: flow) set
+        by hand: shape -1, 0, 4..6 (per-link form) or 7..9 (plaquette form, units
+        of 4, 8, 16 planes)."""
+        self._ck(self.L.tcym_batch_set_schedule(self.h, mode, shape, blocks_per_cu, 1 if xcd_split else 0))
 
     # ---- timing ----
     def timing(self, enable=True):

@@ -44,6 +44,15 @@ struct KGeom {
     FastDiv dnz, dtcx;   // division by nz and tcx (patch walk)
 };
 
+// Rows of a block of nt threads in the plaquette form of the update: the
+// largest divisor py of ny with 2 <= py and nx py <= nt (ny itself when the
+// whole plane fits), 0 if there is none.
+inline int plaq_rows(int nx, int ny, int nt) {
+    for (int py = nt/nx; py >= 2; py--)
+        if (ny % py == 0) return py;
+    return 0;
+}
+
 // One lattice of a batch (device table).  Each pass over a batch has its own
 // tiling: `gu`/`utile0` for the leapfrog and Langevin updates and `gf`/`ftile0`
 // for the flow (the autotuner picks each walk shape), `gr`/`ltile0`/`stile0`
@@ -57,6 +66,10 @@ struct BatchEns {
     uint32_t ftile0;           // first tile of its flow pass
     uint32_t ltile0;           // first tile of its measurement pass over links
     uint32_t stile0;           // first tile of its measurement pass over sites
+    // plaquette form of the update (u) and flow (f) passes, where the
+    // autotuner picked it: rows and planes of a unit, bands along y (the
+    // pass's tile table then lists units, from utile0 / ftile0)
+    int32_t upy, upz, unyb, fpy, fpz, fnyb;
 };
 
 // device tables of a batch handed to the kernels (owned by the batch object)
@@ -71,6 +84,7 @@ struct BatchDev {
     int grid_flow;                     // persistent grid of the flow kernel
     int grid_measure;                  // persistent grid of the measurement kernels
     const uint32_t* step;              // device step counter (Philox step of step offset 0)
+    bool plaq_update, plaq_flow;       // plaquette form of the pass (its table lists plaquette units)
 };
 
 struct Backend {

@@ -53,6 +53,23 @@ __global__ void k_advance(uint32_t* __restrict__ counters, uint32_t steps) {
 
 KGeom kgeom(const Geom& g, const WalkShape& w) { return make_kgeom(g, w); }
 
+// Shapes of an update pass: -1 the canonical walk, 0 the linear walk, 4..6
+// patch walks of the per-link form; 7, 8, 9 the plaquette form (FP32) with
+// units of 4, 8 or 16 planes.
+constexpr int kShapePlaq = 7;
+constexpr int kPlaqBlock = 256;      // threads of a plaquette block (kBatchPlaqThreads)
+int plaq_planes(int shape) { return 2 << (shape - kShapePlaq); }
+// rows of a plaquette unit of lattice q, 0 if the form does not take it
+// (rows of more than half a block need the whole plane in one block)
+int plaq_rows_of(const Geom& q) {
+    const int py = tcym::plaq_rows(q.n[0], q.n[1], kPlaqBlock);
+    return py && (py == q.n[1] || 2*q.n[0] <= kPlaqBlock) ? py : 0;
+}
+long plaq_units(const Geom& q, int shape) {
+    const int py = plaq_rows_of(q), pz = plaq_planes(shape);
+    return (long)(q.n[1]/py)*((q.n[2] + pz - 1)/pz);
+}
+
 // walk shape of the update kernels: -1 the canonical one, 0 linear, 4..6 patches
 WalkShape update_shape(const Geom& g, int shape) {
     if (shape < 0) return tcym::walk_shape(g);
@@ -137,6 +154,8 @@ struct tcym_batch : tcym::DeviceQueue {
         d.grid_flow = grid_for(n_ftiles, per_cu[1], xcd_split[1]);
         d.grid_measure = grid_for(std::max(n_ltiles, n_stiles), 8, true);
         d.step = counters;
+        d.plaq_update = shape[0] >= kShapePlaq;
+        d.plaq_flow = shape[1] >= kShapePlaq;
         return d;
     }
     void drop_graphs() {
@@ -148,12 +167,22 @@ struct tcym_batch : tcym::DeviceQueue {
     }
     // tile tables for the walk shapes su (leapfrog, Langevin) and sf (flow)
     // (measurement passes always use the canonical shape of each lattice)
+    // every lattice takes the plaquette form (FP32 storage only)
+    bool plaq_ok() const {
+        if (be->elsize != 4) return false;
+        for (const Geom& q : geo)
+            if (!plaq_rows_of(q)) return false;
+        return true;
+    }
     int build_tables(int su, int sf) {
+        if ((su >= kShapePlaq || sf >= kShapePlaq) && !plaq_ok())
+            return fail(TCYM_E_UNSUPPORTED, "the plaquette form needs FP32 storage and rows that fit its blocks");
         std::vector<uint32_t> ut, ft, lt, st, l0, ln, s0, sn;
         ens.assign(count, BatchEns{});
         for (int e = 0; e < count; e++) {
             const Geom& q = geo[e];
-            const WalkShape wu = update_shape(q, su), wf = update_shape(q, sf), wr = tcym::walk_shape(q);
+            const WalkShape wu = update_shape(q, su < kShapePlaq ? su : -1), wf = update_shape(q, sf < kShapePlaq ? sf : -1),
+                            wr = tcym::walk_shape(q);
             BatchEns& b = ens[e];
             b.gu = kgeom(q, wu);
             b.gf = kgeom(q, wf);
@@ -167,8 +196,20 @@ struct tcym_batch : tcym::DeviceQueue {
             b.stile0 = (uint32_t)st.size();
             l0.push_back(b.ltile0); ln.push_back((uint32_t)(3*wr.blocks));
             s0.push_back(b.stile0); sn.push_back((uint32_t)wr.blocks);
-            ut.insert(ut.end(), (size_t)(3*wu.blocks), (uint32_t)e);
-            ft.insert(ft.end(), (size_t)(3*wf.blocks), (uint32_t)e);
+            // a pass's table lists walk tiles or, in the plaquette form, units
+            // (band-major within a slab: unit k is band k % nyb of slab k / nyb)
+            if (su >= kShapePlaq) {
+                b.upy = plaq_rows_of(q); b.upz = plaq_planes(su); b.unyb = q.n[1]/b.upy;
+                ut.insert(ut.end(), (size_t)plaq_units(q, su), (uint32_t)e);
+            } else {
+                ut.insert(ut.end(), (size_t)(3*wu.blocks), (uint32_t)e);
+            }
+            if (sf >= kShapePlaq) {
+                b.fpy = plaq_rows_of(q); b.fpz = plaq_planes(sf); b.fnyb = q.n[1]/b.fpy;
+                ft.insert(ft.end(), (size_t)plaq_units(q, sf), (uint32_t)e);
+            } else {
+                ft.insert(ft.end(), (size_t)(3*wf.blocks), (uint32_t)e);
+            }
             lt.insert(lt.end(), (size_t)(3*wr.blocks), (uint32_t)e);
             st.insert(st.end(), (size_t)wr.blocks, (uint32_t)e);
         }
@@ -178,8 +219,10 @@ struct tcym_batch : tcym::DeviceQueue {
         size_t ucap = 0;
         for (const Geom& q : geo) {
             long m = 0;
-            for (int s : {-1, 0, 4, 5, 6}) m = std::max(m, update_shape(q, s).blocks);
-            ucap += (size_t)(3*m);
+            for (int s : {-1, 0, 4, 5, 6}) m = std::max(m, 3*update_shape(q, s).blocks);
+            if (plaq_rows_of(q))
+                for (int s = kShapePlaq; s < kShapePlaq + 3; s++) m = std::max(m, plaq_units(q, s));
+            ucap += (size_t)m;
         }
         int r;
         if ((r = alloc((void**)&utiles, sizeof(uint32_t)*ucap, "hipMalloc(tiles)"))) return r;
@@ -651,7 +694,10 @@ int tcym_batch_tune(tcym_batch* b, int mode, double dt, int reps, int choice[3],
             }
         }
     };
-    for (int s : {-1, 0, 4, 5, 6}) {
+    std::vector<int> shapes = {-1, 0, 4, 5, 6};
+    if (b->plaq_ok())
+        for (int s = kShapePlaq; s < kShapePlaq + 3; s++) shapes.push_back(s);
+    for (int s : shapes) {
         if ((r = pass == 0 ? b->build_tables(s, shape0[1]) : b->build_tables(shape0[0], s))) break;
         for (int pc : {2, 4, 8, 16}) {
             for (bool xs : {true, false}) {
@@ -683,6 +729,20 @@ int tcym_batch_tune(tcym_batch* b, int mode, double dt, int reps, int choice[3],
     if (choice) { choice[0] = bs; choice[1] = bp; choice[2] = bx ? 1 : 0; }
     if (us) *us = best;
     return TCYM_OK;
+}
+
+int tcym_batch_set_schedule(tcym_batch* b, int mode, int shape, int blocks_per_cu, int xcd_split) {
+    if (!b) return TCYM_E_ARG;
+    if (mode != 0 && mode != 1 && mode != 3) return b->fail(TCYM_E_ARG, "schedule: mode 0, 1 or 3");
+    const bool plaq = shape >= kShapePlaq && shape < kShapePlaq + 3;
+    if (!(shape == -1 || shape == 0 || (shape >= 4 && shape <= 6) || plaq) || blocks_per_cu < 1 || blocks_per_cu > 64)
+        return b->fail(TCYM_E_ARG, "schedule: shape -1, 0, 4..6 or 7..9, 1..64 blocks per compute unit");
+    const int pass = mode == 3 ? 1 : 0;
+    int r;
+    if ((r = b->resolve())) return r;
+    b->per_cu[pass] = blocks_per_cu;
+    b->xcd_split[pass] = xcd_split != 0;
+    return pass == 0 ? b->build_tables(shape, b->shape[1]) : b->build_tables(b->shape[0], shape);
 }
 
 // ---------------------------------------------------------------- timing
