@@ -98,7 +98,7 @@ int tcym_batch_unitarity(tcym_batch *b, int u, double *out);
 /* Autotuner of the update kernels: times every candidate shape (per-link
  * form: canonical, linear and 64/32/16-wide patch walks, shapes -1, 0, 4..6;
  * FP32 batches whose rows fit a 256-thread block also the plaquette form with
- * units of 4, 8 or 16 planes, shapes 7..9) and persistent grid size on
+ * units of 2, 4 or 8 planes, shapes 7..9) and persistent grid size on
  * scratch copies of the fields and keeps the fastest for one pass: mode 0 or
  * 1 tunes the leapfrog and Langevin updates (repeated launches on one
  * input), mode 3 the flow (sequences of flow steps alternating between two
