@@ -93,6 +93,12 @@ template <typename T> struct SU2 {
     }
     // o[a] = c ReTr(i t^a M) = -c m_a
     __device__ static __forceinline__ void proj(T c, const Mat& M, T* o) { o[0] = -c*M.v[0]; o[1] = -c*M.v[1]; o[2] = -c*M.v[2]; }
+    // o[a] = c ReTr(i t^a A B)
+    __device__ static __forceinline__ void proj_mul(T c, const Mat& A, const Mat& B, T* o) {
+        o[0] = -c*(A.v[3]*B.v[0] + B.v[3]*A.v[0] - (A.v[1]*B.v[2] - A.v[2]*B.v[1]));
+        o[1] = -c*(A.v[3]*B.v[1] + B.v[3]*A.v[1] - (A.v[2]*B.v[0] - A.v[0]*B.v[2]));
+        o[2] = -c*(A.v[3]*B.v[2] + B.v[3]*A.v[2] - (A.v[0]*B.v[1] - A.v[1]*B.v[0]));
+    }
     // o[a] += c ReTr(i t^a A B): only the vector part of the product
     __device__ static __forceinline__ void proj_mul_acc(T c, const Mat& A, const Mat& B, T* o) {
         o[0] -= c*(A.v[3]*B.v[0] + B.v[3]*A.v[0] - (A.v[1]*B.v[2] - A.v[2]*B.v[1]));
@@ -157,7 +163,11 @@ template <typename T> struct SU2 {
         o[2] = s*E[2] + T(2)*V.v[2]*ve + T(2)*w*x2;
     }
     __device__ static __forceinline__ void inv_adj_act(const Mat& V, const T* E, T* o) { adj_act(dag(V), E, o); }
-    __device__ static __forceinline__ void ad_dag(const Mat& V, const T* f, T* o) { adj_act(dag(V), f, o); }
+    template <bool NEG = false>
+    __device__ static __forceinline__ void ad_dag(const Mat& V, const T* f, T* o) {
+        adj_act(dag(V), f, o);
+        if constexpr (NEG) { o[0] = -o[0]; o[1] = -o[1]; o[2] = -o[2]; }
+    }
     __device__ static __forceinline__ T unitarity_norm(const Mat& U) {
         Mat C = mul_nd(U, U);
         return dsqrt(C.v[0]*C.v[0] + C.v[1]*C.v[1] + C.v[2]*C.v[2] + (C.v[3] - T(1))*(C.v[3] - T(1)));
@@ -283,6 +293,8 @@ template <typename T> struct SU3 {
     __device__ static __forceinline__ T retr_1m(const Mat& M) { return T(3) - M.r[0] - M.r[4] - M.r[8]; }
     // o[a] += c ReTr(i t^a A B), evaluating only the entries of A B the
     // projection reads: the six off-diagonal ones and the diagonal's imaginary parts
+    // (ACC false: o[a] = c ReTr(i t^a A B))
+    template <bool ACC = true>
     __device__ static __forceinline__ void proj_mul_acc(T c, const Mat& A, const Mat& B, T* o) {
         auto ent = [&](int row, int col, T& re, T& im) {
             T sr = T(0), si = T(0);
@@ -309,15 +321,12 @@ template <typename T> struct SU3 {
         ent(2, 1, r21, i21); ent(1, 2, r12, i12);
         const T i00 = dim(0), i11 = dim(1), i22 = dim(2);
         const T h = T(-0.5)*c, s3 = T(0.57735026918962576450914878050196);
-        o[0] += h*(i01 + i10);
-        o[1] += h*(r01 - r10);
-        o[2] += h*(i00 - i11);
-        o[3] += h*(i02 + i20);
-        o[4] += h*(r02 - r20);
-        o[5] += h*(i12 + i21);
-        o[6] += h*(r12 - r21);
-        o[7] += h*(i00 + i11 - T(2)*i22)*s3;
+        const T p[8] = {h*(i01 + i10), h*(r01 - r10), h*(i00 - i11), h*(i02 + i20),
+                        h*(r02 - r20), h*(i12 + i21), h*(r12 - r21), h*(i00 + i11 - T(2)*i22)*s3};
+#pragma unroll
+        for (int a = 0; a < 8; a++) o[a] = ACC ? o[a] + p[a] : p[a];
     }
+    __device__ static __forceinline__ void proj_mul(T c, const Mat& A, const Mat& B, T* o) { proj_mul_acc<false>(c, A, B, o); }
     // a A + b B
     __device__ static __forceinline__ Mat lin(T a, const Mat& A, T b, const Mat& B) {
         Mat C;
@@ -341,28 +350,28 @@ template <typename T> struct SU3 {
         A.r[7] = A.r[5]; A.i[7] = -A.i[5];      // (1,2)
         return A;
     }
-    // Q^2 of a hermitian Q with a real diagonal: the diagonal and the lower
-    // triangle, the upper triangle by conjugation (half a general product)
+    // Q^2 of a hermitian Q with a real diagonal d and lower triangle
+    // q10, q20, q21 (upper triangle by conjugation): the diagonal and the
+    // lower triangle of Q^2, no product with a vanishing imaginary part
+    //   (Q^2)_10 = q10 (d0 + d1) + conj(q21) q20
+    //   (Q^2)_20 = q20 (d0 + d2) + q21 q10
+    //   (Q^2)_21 = q21 (d1 + d2) + q20 conj(q10)
     __device__ static __forceinline__ Mat herm_sq(const Mat& Q) {
         Mat P;
-        P.r[0] = Q.r[0]*Q.r[0] + (Q.r[3]*Q.r[3] + Q.i[3]*Q.i[3]) + (Q.r[6]*Q.r[6] + Q.i[6]*Q.i[6]);
-        P.r[4] = (Q.r[1]*Q.r[1] + Q.i[1]*Q.i[1]) + Q.r[4]*Q.r[4] + (Q.r[7]*Q.r[7] + Q.i[7]*Q.i[7]);
-        P.r[8] = (Q.r[2]*Q.r[2] + Q.i[2]*Q.i[2]) + (Q.r[5]*Q.r[5] + Q.i[5]*Q.i[5]) + Q.r[8]*Q.r[8];
+        const T d0 = Q.r[0], d1 = Q.r[4], d2 = Q.r[8];
+        const T ar = Q.r[1], ai = Q.i[1], br = Q.r[2], bi = Q.i[2], cr = Q.r[5], ci = Q.i[5];   // q10, q20, q21
+        const T na = fmat(ar, ar, ai*ai), nb = fmat(br, br, bi*bi), nc = fmat(cr, cr, ci*ci);
+        P.r[0] = fmat(d0, d0, na + nb);
+        P.r[4] = fmat(d1, d1, na + nc);
+        P.r[8] = fmat(d2, d2, nb + nc);
         P.i[0] = T(0); P.i[4] = T(0); P.i[8] = T(0);
-        // (r, c) = sum_k Q(r, k) Q(k, c), element (r, c) at r + 3 c
-        auto el = [&](int r, int c, T& re, T& im) {
-            re = T(0); im = T(0);
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const T ar = Q.r[r + 3*k], ai = (r == k) ? T(0) : Q.i[r + 3*k];
-                const T br = Q.r[k + 3*c], bi = (k == c) ? T(0) : Q.i[k + 3*c];
-                re += ar*br - ai*bi;
-                im += ar*bi + ai*br;
-            }
-        };
-        el(1, 0, P.r[1], P.i[1]);
-        el(2, 0, P.r[2], P.i[2]);
-        el(2, 1, P.r[5], P.i[5]);
+        const T s01 = d0 + d1, s02 = d0 + d2, s12 = d1 + d2;
+        P.r[1] = fmat(ar, s01, fmat(cr, br, ci*bi));
+        P.i[1] = fmat(ai, s01, fmat(cr, bi, -(ci*br)));
+        P.r[2] = fmat(br, s02, fmat(cr, ar, -(ci*ai)));
+        P.i[2] = fmat(bi, s02, fmat(cr, ai, ci*ar));
+        P.r[5] = fmat(cr, s12, fmat(br, ar, bi*ai));
+        P.i[5] = fmat(ci, s12, fmat(bi, ar, -(br*ai)));
         P.r[3] = P.r[1]; P.i[3] = -P.i[1];
         P.r[6] = P.r[2]; P.i[6] = -P.i[2];
         P.r[7] = P.r[5]; P.i[7] = -P.i[5];
@@ -396,10 +405,15 @@ template <typename T> struct SU3 {
     // triangle of X', o_3 and o_8 come from its diagonal, and X'(2,2) =
     // -(X'(0,0) + X'(1,1)) since H' is traceless, so only columns 0 and 1
     // of Y = H' V are formed (60 instead of 90 multiply-adds)
-    __device__ static __forceinline__ void ad_dag(const Mat& V, const T* f, T* o) {
+    // NEG: o = -(V^+ H V), the sign taken into H' (free operand negations)
+    template <bool NEG = false>
+    __device__ static __forceinline__ void ad_dag(const Mat& V, const T* f0, T* o) {
         const T s3 = T(0.57735026918962576450914878050196);
+        T f[8];
+#pragma unroll
+        for (int a = 0; a < 8; a++) f[a] = NEG ? -f0[a] : f0[a];
         // H' = f^a lambda_a: real diagonal, lower triangle (1,0), (2,0), (2,1)
-        const T h00 = f[2] + f[7]*s3, h11 = -f[2] + f[7]*s3, h22 = T(-2)*f[7]*s3;
+        const T h00 = fmat(f[7], s3, f[2]), h11 = fmat(f[7], s3, -f[2]), h22 = T(-2)*f[7]*s3;
         const T hr[3][3] = {{h00, f[0], f[3]}, {f[0], h11, f[5]}, {f[3], f[5], h22}};
         const T hi[3][3] = {{T(0), -f[1], -f[4]}, {f[1], T(0), -f[6]}, {f[4], f[6], T(0)}};
         T Yr[3][2], Yi[3][2];
@@ -465,17 +479,22 @@ template <typename T> struct SU3 {
         int s = 0;
         T cs1 = c1, cs0 = c0, sc = T(1);
         while (cs1 > T(0.25) && s < 40) { cs1 *= T(0.25); cs0 *= T(0.125); sc *= T(0.5); s++; }
-        // Taylor series in the (1,Q,Q^2) basis
-        T pa = T(1), pb = T(0), pd = T(0);             // Q^k = pa + pb Q + pd Q^2
-        T f0r = T(0), f0i = T(0), f1r = T(0), f1i = T(0), f2r = T(0), f2i = T(0);
+        // Taylor series in the (1,Q,Q^2) basis.  Terms 0 .. 4 written out
+        // (Q^0 .. Q^4 = 1, Q, Q^2, cs0 + cs1 Q, cs0 Q + cs1 Q^2), so that no
+        // coefficient is multiplied by a vanishing basis component
+        T f0r = T(1), f0i = T(-kInvFact[3])*cs0, f1r = T(kInvFact[4])*cs0;
+        T f1i = fmat(T(-kInvFact[3]), cs1, T(1)), f2r = fmat(T(kInvFact[4]), cs1, T(-kInvFact[2])), f2i = T(0);
+        T pa = cs0*cs1, pb = cs1*cs1, pd = cs0;         // Q^5 = pa + pb Q + pd Q^2
 #pragma unroll
-        for (int k = 0; k <= Lit<T>::exp_terms; k++) {
+        for (int k = 5; k <= Lit<T>::exp_terms; k++) {
             const T fact = T(kInvFact[k]);
             const T w = ((k >> 1) & 1) ? -fact : fact;   // i^k = (+1, +i, -1, -i)
-            if (k & 1) { f0i += w*pa; f1i += w*pb; f2i += w*pd; }
-            else       { f0r += w*pa; f1r += w*pb; f2r += w*pd; }
-            const T na = cs0*pd, nb = pa + cs1*pd, nd = pb;
-            pa = na; pb = nb; pd = nd;
+            if (k & 1) { f0i = fmat(w, pa, f0i); f1i = fmat(w, pb, f1i); f2i = fmat(w, pd, f2i); }
+            else       { f0r = fmat(w, pa, f0r); f1r = fmat(w, pb, f1r); f2r = fmat(w, pd, f2r); }
+            if (k < Lit<T>::exp_terms) {
+                const T na = cs0*pd, nb = fmat(cs1, pd, pa), nd = pb;
+                pa = na; pb = nb; pd = nd;
+            }
         }
         for (int k = 0; k < s; k++) {
             // p^2 reduced with Q^3 = cs1 Q + cs0, Q^4 = cs1 Q^2 + cs0 Q
@@ -496,12 +515,24 @@ template <typename T> struct SU3 {
         f2r *= sc*sc; f2i *= sc*sc;
         const Mat Q2 = herm_sq(Q);
         Mat U;
+        // diagonal: Q and Q^2 real there
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
-            U.r[k] = f1r*Q.r[k] - f1i*Q.i[k] + f2r*Q2.r[k] - f2i*Q2.i[k];
-            U.i[k] = f1r*Q.i[k] + f1i*Q.r[k] + f2r*Q2.i[k] + f2i*Q2.r[k];
+        for (int k = 0; k < 9; k += 4) {
+            U.r[k] = fmat(f1r, Q.r[k], fmat(f2r, Q2.r[k], f0r));
+            U.i[k] = fmat(f1i, Q.r[k], fmat(f2i, Q2.r[k], f0i));
         }
-        U.r[0] += f0r; U.i[0] += f0i; U.r[4] += f0r; U.i[4] += f0i; U.r[8] += f0r; U.i[8] += f0i;
+        // off-diagonal pairs (lower l, upper u): Q_u = conj(Q_l), Q2_u = conj(Q2_l),
+        // so U_l = (A - B) + i (C + D) and U_u = (A + B) + i (C - D) with
+        // A = f1r q.r + f2r p.r, B = f1i q.i + f2i p.i, C = f1i q.r + f2i p.r, D = f1r q.i + f2r p.i
+        constexpr int lo[3] = {1, 2, 5}, up[3] = {3, 6, 7};
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int l = lo[j], u = up[j];
+            const T a = fmat(f1r, Q.r[l], f2r*Q2.r[l]), b = fmat(f1i, Q.i[l], f2i*Q2.i[l]);
+            const T c = fmat(f1i, Q.r[l], f2i*Q2.r[l]), d = fmat(f1r, Q.i[l], f2r*Q2.i[l]);
+            U.r[l] = a - b; U.r[u] = a + b;
+            U.i[l] = c + d; U.i[u] = c - d;
+        }
         return U;
     }
 
@@ -544,22 +575,30 @@ template <typename T> struct SU3 {
         R c0r[3], c0i[3], c1r[3], c1i[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) { c0r[j] = U.r[j]; c0i[j] = U.i[j]; c1r[j] = U.r[3 + j]; c1i[j] = U.i[3 + j]; }
-        R n = 0;
+        // sums as single multiply-add chains
+        R n = c0r[0]*c0r[0];
+        n = fmat(c0i[0], c0i[0], n);
 #pragma unroll
-        for (int j = 0; j < 3; j++) n += c0r[j]*c0r[j] + c0i[j]*c0i[j];
+        for (int j = 1; j < 3; j++) { n = fmat(c0r[j], c0r[j], n); n = fmat(c0i[j], c0i[j], n); }
         n = rsqrt_reunit<T>(n);
 #pragma unroll
         for (int j = 0; j < 3; j++) { c0r[j] *= n; c0i[j] *= n; }
-        R pr = 0, pi = 0;   // <c0, c1>
+        R pr = c0r[0]*c1r[0], pi = c0r[0]*c1i[0];   // <c0, c1>
+        pr = fmat(c0i[0], c1i[0], pr); pi = fmat(-c0i[0], c1r[0], pi);
 #pragma unroll
-        for (int j = 0; j < 3; j++) { pr += c0r[j]*c1r[j] + c0i[j]*c1i[j]; pi += c0r[j]*c1i[j] - c0i[j]*c1r[j]; }
-        n = 0;
+        for (int j = 1; j < 3; j++) {
+            pr = fmat(c0r[j], c1r[j], pr); pr = fmat(c0i[j], c1i[j], pr);
+            pi = fmat(c0r[j], c1i[j], pi); pi = fmat(-c0i[j], c1r[j], pi);
+        }
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            c1r[j] -= pr*c0r[j] - pi*c0i[j];
-            c1i[j] -= pr*c0i[j] + pi*c0r[j];
-            n += c1r[j]*c1r[j] + c1i[j]*c1i[j];
+            c1r[j] = fmat(-pr, c0r[j], fmat(pi, c0i[j], c1r[j]));
+            c1i[j] = fmat(-pr, c0i[j], fmat(-pi, c0r[j], c1i[j]));
         }
+        n = c1r[0]*c1r[0];
+        n = fmat(c1i[0], c1i[0], n);
+#pragma unroll
+        for (int j = 1; j < 3; j++) { n = fmat(c1r[j], c1r[j], n); n = fmat(c1i[j], c1i[j], n); }
         n = rsqrt_reunit<T>(n);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
