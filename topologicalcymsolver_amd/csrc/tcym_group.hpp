@@ -462,20 +462,16 @@ template <typename T> struct SU3 {
     // the same basis.
     __device__ static __forceinline__ Mat expi(T c, const T* al) {
         const Mat Q = alg_matrix(c, al);
-        // c1 = tr(Q^2)/2 = (c^2/4) sum_a al_a^2, since tr(t^a t^b) = delta_ab/2
-        T c1 = T(0);
-#pragma unroll
-        for (int a = 0; a < 8; a++) c1 = fmat(al[a], al[a], c1);
-        c1 *= T(0.25)*c*c;
+        // Q^2 first: c1 = tr(Q^2)/2 and, Q being traceless, c0 = det Q =
+        // tr(Q^3)/3 = (sum_i Q_ii (Q^2)_ii + 2 sum_{i>j} Re(Q_ij conj((Q^2)_ij)))/3
+        const Mat Q2 = herm_sq(Q);
+        const T c1 = T(0.5)*(Q2.r[0] + Q2.r[4] + Q2.r[8]);
         if (c1 == T(0)) return unit();
-        // det of hermitian Q (real)
-        T m0 = Q.r[4]*Q.r[8] - (Q.r[7]*Q.r[7] + Q.i[7]*Q.i[7]);          // Q11 Q22 - |Q12|^2
-        // Q01*(Q12 Q20 - Q10 Q22) + Q02*(Q10 Q21 - Q11 Q20), real part
-        T ar = Q.r[7]*Q.r[2] - Q.i[7]*Q.i[2] - Q.r[1]*Q.r[8];
-        T ai = Q.r[7]*Q.i[2] + Q.i[7]*Q.r[2] - Q.i[1]*Q.r[8];
-        T br = Q.r[1]*Q.r[5] - Q.i[1]*Q.i[5] - Q.r[4]*Q.r[2];
-        T bi = Q.r[1]*Q.i[5] + Q.i[1]*Q.r[5] - Q.r[4]*Q.i[2];
-        T c0 = Q.r[0]*m0 + (Q.r[3]*ar - Q.i[3]*ai) + (Q.r[6]*br - Q.i[6]*bi);
+        T t3 = fmat(Q.r[1], Q2.r[1], Q.i[1]*Q2.i[1]);
+        t3 = fmat(Q.r[2], Q2.r[2], t3); t3 = fmat(Q.i[2], Q2.i[2], t3);
+        t3 = fmat(Q.r[5], Q2.r[5], t3); t3 = fmat(Q.i[5], Q2.i[5], t3);
+        const T d3 = fmat(Q.r[0], Q2.r[0], fmat(Q.r[4], Q2.r[4], Q.r[8]*Q2.r[8]));
+        const T c0 = fmat(T(2), t3, d3)*T(1.0/3.0);
         int s = 0;
         T cs1 = c1, cs0 = c0, sc = T(1);
         while (cs1 > T(0.25) && s < 40) { cs1 *= T(0.25); cs0 *= T(0.125); sc *= T(0.5); s++; }
@@ -510,10 +506,8 @@ template <typename T> struct SU3 {
             f0r = n0r; f0i = n0i; f1r = n1r; f1i = n1i; f2r = n2r; f2i = n2i;
         }
         // U = f0 + f1 Qs + f2 Qs^2 with Qs = sc Q, as f0 + (sc f1) Q + (sc^2 f2) Q^2
-        // (Q hermitian: its square from one triangle)
         f1r *= sc; f1i *= sc;
         f2r *= sc*sc; f2i *= sc*sc;
-        const Mat Q2 = herm_sq(Q);
         Mat U;
         // diagonal: Q and Q^2 real there
 #pragma unroll
