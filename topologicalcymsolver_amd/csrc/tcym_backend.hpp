@@ -70,6 +70,7 @@ struct BatchEns {
     // autotuner picked it: rows and planes of a unit, bands along y (the
     // pass's tile table then lists units, from utile0 / ftile0)
     int32_t upy, upz, unyb, fpy, fpz, fnyb;
+    FastDiv udnyb, fdnyb;      // division by unyb / fnyb (a unit's slab and band)
 };
 
 // device tables of a batch handed to the kernels (owned by the batch object)

@@ -55,7 +55,7 @@ KGeom kgeom(const Geom& g, const WalkShape& w) { return make_kgeom(g, w); }
 
 // Shapes of an update pass: -1 the canonical walk, 0 the linear walk, 4..6
 // patch walks of the per-link form; 7, 8, 9 the plaquette form (FP32) with
-// units of 4, 8 or 16 planes.
+// units of 2, 4 or 8 planes.
 constexpr int kShapePlaq = 7;
 constexpr int kPlaqBlock = 256;      // threads of a plaquette block (kBatchPlaqThreads)
 int plaq_planes(int shape) { return 2 << (shape - kShapePlaq); }
@@ -200,12 +200,14 @@ struct tcym_batch : tcym::DeviceQueue {
             // (band-major within a slab: unit k is band k % nyb of slab k / nyb)
             if (su >= kShapePlaq) {
                 b.upy = plaq_rows_of(q); b.upz = plaq_planes(su); b.unyb = q.n[1]/b.upy;
+                b.udnyb = tcym::fast_div((uint32_t)b.unyb);
                 ut.insert(ut.end(), (size_t)plaq_units(q, su), (uint32_t)e);
             } else {
                 ut.insert(ut.end(), (size_t)(3*wu.blocks), (uint32_t)e);
             }
             if (sf >= kShapePlaq) {
                 b.fpy = plaq_rows_of(q); b.fpz = plaq_planes(sf); b.fnyb = q.n[1]/b.fpy;
+                b.fdnyb = tcym::fast_div((uint32_t)b.fnyb);
                 ft.insert(ft.end(), (size_t)plaq_units(q, sf), (uint32_t)e);
             } else {
                 ft.insert(ft.end(), (size_t)(3*wf.blocks), (uint32_t)e);
