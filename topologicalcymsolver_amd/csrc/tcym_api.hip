@@ -111,6 +111,14 @@ struct tcym_ctx : tcym::DeviceQueue {
     double* E64 = nullptr;
     double* map = nullptr;
     void* Bc = nullptr;          // field strength of one link slot (tcym_evolve_ncs, within a call)
+    // field strength B of the cooling method's U_OLD, formed by the previous
+    // tcym_cooling_update for its U_NEW and valid while U_OLD is untouched:
+    // uver counts the accesses to each link slot (useU), bcool_ver is U_OLD's
+    // count when B was kept
+    void* Bcool = nullptr;
+    uint64_t uver[TCYM_NUM_U] = {};
+    uint64_t bcool_ver = 0;
+    int bcool_ok = 0, bcool_lev = 0;
 
     // Resolution level of each slot: 0 is the full lattice; level l is the
     // lattice blocked l times (extent N >> l, spacing a << l), which a
@@ -146,6 +154,7 @@ struct tcym_ctx : tcym::DeviceQueue {
 
     int useU(int slot) {
         if (slot < 0 || slot >= TCYM_NUM_U) return fail(TCYM_E_ARG, "link slot out of range");
+        uver[slot]++;   // every reader and writer of a slot passes here
         return alloc(&U[slot], (size_t)link_elems()*be->elsize, "hipMalloc(links)");
     }
     int useE(int slot) {
@@ -189,6 +198,20 @@ struct tcym_ctx : tcym::DeviceQueue {
         double v;
         int r = edotb(u, e, 1, &v);
         if (r) return r;
+        *out = 0.125*v/(M_PI*M_PI);
+        return TCYM_OK;
+    }
+    // NCsDot with the field strength of u kept in Bcool (bmode 1: formed and
+    // stored; 2: read from Bcool, where a bmode-1 call on the same links left
+    // it); the same sum to the bit as ncsdot
+    int ncsdot_kept(int u, int e, int bmode, double* out) {
+        int r;
+        Geom q;
+        if ((r = useU(u)) || (r = useE(e)) || (r = at(q, u, e))) return r;
+        if ((r = alloc(&Bcool, (size_t)field_elems()*be->elsize, "hipMalloc(field strength)"))) return r;
+        const int mx[1] = {0};
+        double v;
+        if ((r = reduce(1, mx, [&] { return be->edotb_cached(L(), q, U[u], E[e], part, Bcool, bmode); }, &v))) return r;
         *out = 0.125*v/(M_PI*M_PI);
         return TCYM_OK;
     }
@@ -651,20 +674,25 @@ int tcym_geodesic(tcym_ctx* c, int uold, int unew, int umid, int emid, double cc
     c->be->geodesic(c->L(), q, cc, c->U[uold], c->U[unew], c->U[umid], c->E[emid]);
     return c->sync();
 }
-int tcym_delta_ncs_cooling(tcym_ctx* c, int uold, int unew, double* out) {
-    if (!c || !out) return TCYM_E_ARG;
+// CoolingMethod::GetDeltaNCs(1): Simpson (1,4,1)/6 over the geodesic
+// (TOPOLOGY/CoolingMethod.cpp:80-121).  kept_old: B of uold is in Bcool;
+// keep_new: B of unew is left in Bcool
+static int delta_ncs(tcym_ctx* c, int uold, int unew, bool kept_old, bool keep_new, double* out) {
     int r = tcym_geodesic(c, uold, unew, TCYM_U_MID, TCYM_E_MID, 0.5);
     if (r) return r;
     double v = 0.0, w = 0.0, x;
-    // CoolingMethod::GetDeltaNCs(1): Simpson (1,4,1)/6 (TOPOLOGY/CoolingMethod.cpp:80-121)
     if ((r = c->ncsdot(TCYM_U_MID, TCYM_E_MID, &x))) return r;
     v += 4.0*x; w += 4.0;
-    if ((r = c->ncsdot(uold, TCYM_E_MID, &x))) return r;
+    if ((r = kept_old ? c->ncsdot_kept(uold, TCYM_E_MID, 2, &x) : c->ncsdot(uold, TCYM_E_MID, &x))) return r;
     v += x; w += 1.0;
-    if ((r = c->ncsdot(unew, TCYM_E_MID, &x))) return r;
+    if ((r = keep_new ? c->ncsdot_kept(unew, TCYM_E_MID, 1, &x) : c->ncsdot(unew, TCYM_E_MID, &x))) return r;
     v += x; w += 1.0;
     *out = v/w;
     return TCYM_OK;
+}
+int tcym_delta_ncs_cooling(tcym_ctx* c, int uold, int unew, double* out) {
+    if (!c || !out) return TCYM_E_ARG;
+    return delta_ncs(c, uold, unew, false, false, out);
 }
 int tcym_cooling_start(tcym_ctx* c, int u_hot, int nsteps, double dtau) {
     if (!c) return TCYM_E_ARG;
@@ -674,14 +702,26 @@ int tcym_cooling_start(tcym_ctx* c, int u_hot, int nsteps, double dtau) {
 }
 int tcym_cooling_update(tcym_ctx* c, int u_hot, int nsteps, double dtau, int measure, double* dncs) {
     if (!c) return TCYM_E_ARG;
+    // U_OLD is the previous call's U_NEW: its field strength, formed then for
+    // the last E.B of the Simpson rule, serves this call's E.B at U_OLD (one
+    // B of the three per measurement saved; the sum is the same to the bit)
+    // unless anything has accessed U_OLD since
+    const bool kept = c->bcool_ok && c->bcool_ver == c->uver[TCYM_U_OLD] && c->bcool_lev == c->ulev[TCYM_U_OLD];
+    c->bcool_ok = 0;
     int r = tcym_cool(c, u_hot, TCYM_U_NEW, nsteps, dtau);
     if (r) return r;
     if (measure) {
         double d;
-        if ((r = tcym_delta_ncs_cooling(c, TCYM_U_OLD, TCYM_U_NEW, &d))) return r;
+        if ((r = delta_ncs(c, TCYM_U_OLD, TCYM_U_NEW, kept, true, &d))) return r;
         if (dncs) *dncs = d;
     }
-    return tcym_copy_links(c, TCYM_U_OLD, TCYM_U_NEW);
+    if ((r = tcym_copy_links(c, TCYM_U_OLD, TCYM_U_NEW))) return r;
+    if (measure) {
+        c->bcool_ok = 1;
+        c->bcool_ver = c->uver[TCYM_U_OLD];
+        c->bcool_lev = c->ulev[TCYM_U_OLD];
+    }
+    return TCYM_OK;
 }
 int tcym_vacuum_ncs(tcym_ctx* c, int u, double* out) {
     if (!c || !out) return TCYM_E_ARG;
