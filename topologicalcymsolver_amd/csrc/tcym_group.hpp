@@ -673,15 +673,35 @@ struct Log3 {
     // the identity but for R_pp = c, R_pq = s, R_qp = -s conj(e),
     // R_qq = c conj(e), e = A_pq/|A_pq|; only columns p, q of A R and V R and
     // rows p, q of R^+ (A R) change
+    // 1/sqrt(x) and 1/x for finite x > 0 (x >= 1 for rcp): the hardware
+    // estimates refined by Newton steps to within a few ulp (a rotation needs
+    // c^2 + s^2 = 1 to rounding, not correctly rounded angles: Jacobi sweeps
+    // converge from any rotation close to the exact one)
+    __device__ static __forceinline__ double rsq(double x) {
+        const double h = 0.5*x;
+        double r = __builtin_amdgcn_rsq(x);
+        r = r*__builtin_fma(-h*r, r, 1.5);
+        return r*__builtin_fma(-h*r, r, 1.5);
+    }
+    __device__ static __forceinline__ double rcp(double x) {
+        double r = __builtin_amdgcn_rcp(x);
+        r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+        return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    }
     template <int p, int q>
     __device__ static __forceinline__ void rot(C* A, C* V) {
         const C apq = A[p + 3*q];
-        const double g = __builtin_sqrt(apq.re*apq.re + apq.im*apq.im);
-        if (g == 0.0) return;
-        const C e = csc(1.0/g, apq), ec = cj(e);
-        const double th = (A[q + 3*q].re - A[p + 3*p].re)/(2.0*g);
-        const double t = (th >= 0 ? 1.0 : -1.0)/(fabs(th) + __builtin_sqrt(th*th + 1.0));
-        const double cc = 1.0/__builtin_sqrt(t*t + 1.0), sn = t*cc;
+        const double g2 = apq.re*apq.re + apq.im*apq.im;
+        if (g2 == 0.0) return;
+        const double rg = rsq(g2);     // 1/|A_pq|
+        const C e = csc(rg, apq), ec = cj(e);
+        const double th = (A[q + 3*q].re - A[p + 3*p].re)*(0.5*rg);
+        // t = sgn(th)/(|th| + sqrt(th^2 + 1)); for |th| > 2^500 (th^2 would
+        // overflow) sqrt(th^2 + 1) = |th| to rounding
+        const double ath = __builtin_fabs(th), q2 = th*th + 1.0;
+        const double den = ath > 0x1p500 ? ath + ath : ath + q2*rsq(q2);
+        const double t = (th >= 0 ? 1.0 : -1.0)*rcp(den);
+        const double cc = rsq(t*t + 1.0), sn = t*cc;
 #pragma unroll
         for (int r = 0; r < 3; r++) {
             const C u = cm(A[r + 3*q], ec), a = A[r + 3*p];
