@@ -10,15 +10,26 @@ namespace tcym {
 // shuffle tree and a fixed 4-wave combine.  Thread 0 returns the result.
 __device__ __forceinline__ double final_reduce_block(const double* __restrict__ part, long nblk, int nq, int q, bool mx) {
     double s = 0.0, c = 0.0;
-    for (long b = threadIdx.x; b < nblk; b += blockDim.x) {
-        const double v = part[b*nq + q];
+    auto add = [&](double v) {
         if (mx) s = fmax(s, v);
         else {
             const double t = s + v;
             c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
             s = t;
         }
+    };
+    // a thread's partials b, b + stride, ... in that order; eight loads in
+    // flight at a time instead of one (the sum waited on each load's latency)
+    const long stride = blockDim.x;
+    long b = threadIdx.x;
+    for (; b + 7*stride < nblk; b += 8*stride) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = part[(b + k*stride)*nq + q];
+#pragma unroll
+        for (int k = 0; k < 8; k++) add(v[k]);
     }
+    for (; b < nblk; b += stride) add(part[b*nq + q]);
     if (!mx) s += c;
     __shared__ double sm[4];
     for (int o = 32; o > 0; o >>= 1) {
