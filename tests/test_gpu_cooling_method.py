@@ -1,13 +1,15 @@
 """GPU: the cooling method's Delta N_CS over several measurement intervals
 (ChernSimonsNumber::CoolingMethod::Update, TOPOLOGY/CoolingMethod.cpp:127-165).
 
-tcym_cooling_update keeps the field strength B of its U_NEW for the next
-call, where the same links are U_OLD, so each measurement after the first
-forms two field strengths instead of three.  The kept B is used only while
-nothing has accessed U_OLD since; these tests check that the kept path gives
-the same sums to the bit as the path that forms all three, that any access
-to U_OLD (a read or a write) between calls makes the next call form B
-again, and that the sequence matches the oracle.
+tcym_delta_ncs_cooling keeps the field strength B of its U_NEW; when the
+links are then copied to U_OLD (tcym_cooling_update does, and so does the
+drop-in's CoolingMethod::Update), the next measurement reads B for its E.B
+at U_OLD, so each measurement after the first forms two field strengths
+instead of three.  The kept B is used only while nothing has accessed those
+links since; these tests check that the kept path gives the same sums to
+the bit as the path that forms all three, that any access to U_OLD (a read
+or a write) between measurements makes the next one form B again, and that
+the sequence matches the oracle.
 """
 import numpy as np
 import pytest
@@ -85,3 +87,30 @@ def test_measurement_sequence_matches_oracle(nc, N):
         d_or = lat.delta_ncs_cooling(u_old, cold.U)
         assert abs(d_gpu - d_or) < 1e-11 * max(1.0, abs(d_or)), (k, d_gpu, d_or)
         u_old = cold.U.copy()
+
+
+@pytest.mark.parametrize("nc,N", CASES)
+def test_kept_field_strength_follows_the_copy(nc, N):
+    # the drop-in's order: cool into U_NEW, Delta N_CS, then U_OLD <- U_NEW
+    _, a = solver(nc, N, 35, T.FP32)
+    _, b = solver(nc, N, 35, T.FP32)
+    da, db = [], []
+    for s in (a, b):
+        s.copy_links(T.U_AUX, T.U_HOT)
+        s.cool_n_save(8, 0.05, u_cold=T.U_NEW)
+        s.copy_links(T.U_OLD, T.U_NEW)
+    for _ in range(3):
+        for s, d in ((a, da), (b, db)):
+            s.dynamics_update(5, 0.05)
+            s.cool_n_save(8, 0.05, u_cold=T.U_NEW)
+            if s is b:
+                s.get_links(T.U_OLD)                # b forms all three B
+            d.append(s.delta_ncs_cooling(T.U_OLD, T.U_NEW))
+            s.copy_links(T.U_OLD, T.U_NEW)
+    assert da == db, (da, db)
+    # a copy into U_OLD from links without a kept B, then a measurement
+    for s in (a, b):
+        s.copy_links(T.U_OLD, T.U_AUX)
+        s.cool_n_save(8, 0.05, u_cold=T.U_NEW)
+    b.unitarity_violation(u=T.U_OLD)
+    assert a.delta_ncs_cooling(T.U_OLD, T.U_NEW) == b.delta_ncs_cooling(T.U_OLD, T.U_NEW)

@@ -111,14 +111,17 @@ struct tcym_ctx : tcym::DeviceQueue {
     double* E64 = nullptr;
     double* map = nullptr;
     void* Bc = nullptr;          // field strength of one link slot (tcym_evolve_ncs, within a call)
-    // field strength B of the cooling method's U_OLD, formed by the previous
-    // tcym_cooling_update for its U_NEW and valid while U_OLD is untouched:
-    // uver counts the accesses to each link slot (useU), bcool_ver is U_OLD's
-    // count when B was kept
+    // Field strength B kept by the cooling measurement (tcym_delta_ncs_cooling)
+    // for the links of slot bk_slot: valid while that slot's access count
+    // uver (bumped by useU, which every reader and writer of a slot passes)
+    // still equals bk_ver.  A copy of those links to another slot moves B's
+    // validity with them (tcym_copy_links), so the E.B at U_OLD of the next
+    // measurement reads the B formed for U_NEW by the previous one.
     void* Bcool = nullptr;
     uint64_t uver[TCYM_NUM_U] = {};
-    uint64_t bcool_ver = 0;
-    int bcool_ok = 0, bcool_lev = 0;
+    uint64_t bk_ver = 0;
+    int bk_slot = -1;
+    bool kept_for(int u) const { return u >= 0 && u == bk_slot && bk_ver == uver[u]; }
 
     // Resolution level of each slot: 0 is the full lattice; level l is the
     // lattice blocked l times (extent N >> l, spacing a << l), which a
@@ -211,7 +214,10 @@ struct tcym_ctx : tcym::DeviceQueue {
         if ((r = alloc(&Bcool, (size_t)field_elems()*be->elsize, "hipMalloc(field strength)"))) return r;
         const int mx[1] = {0};
         double v;
+        bk_slot = -1;
         if ((r = reduce(1, mx, [&] { return be->edotb_cached(L(), q, U[u], E[e], part, Bcool, bmode); }, &v))) return r;
+        bk_slot = u;            // B describes u's links (formed, or read and unchanged)
+        bk_ver = uver[u];
         *out = 0.125*v/(M_PI*M_PI);
         return TCYM_OK;
     }
@@ -331,11 +337,17 @@ int tcym_get_efield(tcym_ctx* c, int slot, double* E) {
 int tcym_copy_links(tcym_ctx* c, int dst, int src) {
     if (!c) return TCYM_E_ARG;
     int r;
+    const bool kept = c->kept_for(src);
     if ((r = c->useU(dst)) || (r = c->useU(src))) return r;
     if (dst == src) return TCYM_OK;
     c->ulev[dst] = c->ulev[src];
     c->be->copy(c->L(), c->link_elems(c->geo(c->ulev[src])), c->U[src], c->U[dst]);
-    return c->sync();
+    if ((r = c->sync())) return r;
+    if (kept) {                 // the kept field strength now describes dst
+        c->bk_slot = dst;
+        c->bk_ver = c->uver[dst];
+    }
+    return TCYM_OK;
 }
 int tcym_copy_efield(tcym_ctx* c, int dst, int src) {
     if (!c) return TCYM_E_ARG;
@@ -675,9 +687,15 @@ int tcym_geodesic(tcym_ctx* c, int uold, int unew, int umid, int emid, double cc
     return c->sync();
 }
 // CoolingMethod::GetDeltaNCs(1): Simpson (1,4,1)/6 over the geodesic
-// (TOPOLOGY/CoolingMethod.cpp:80-121).  kept_old: B of uold is in Bcool;
-// keep_new: B of unew is left in Bcool
-static int delta_ncs(tcym_ctx* c, int uold, int unew, bool kept_old, bool keep_new, double* out) {
+// (TOPOLOGY/CoolingMethod.cpp:80-121).  E.B is linear in E, so the field
+// strength of unew formed here is kept: when the caller then copies unew to
+// uold (CoolingMethod::Update does, :143-165) and touches uold no more, the
+// next call reads it for its E.B at uold instead of forming it (two field
+// strengths per measurement instead of three; the same sums to the bit).
+int tcym_delta_ncs_cooling(tcym_ctx* c, int uold, int unew, double* out) {
+    if (!c || !out) return TCYM_E_ARG;
+    // checked before the geodesic touches uold (and writes the interpolation slot)
+    const bool kept_old = c->kept_for(uold) && uold != TCYM_U_MID;
     int r = tcym_geodesic(c, uold, unew, TCYM_U_MID, TCYM_E_MID, 0.5);
     if (r) return r;
     double v = 0.0, w = 0.0, x;
@@ -685,14 +703,10 @@ static int delta_ncs(tcym_ctx* c, int uold, int unew, bool kept_old, bool keep_n
     v += 4.0*x; w += 4.0;
     if ((r = kept_old ? c->ncsdot_kept(uold, TCYM_E_MID, 2, &x) : c->ncsdot(uold, TCYM_E_MID, &x))) return r;
     v += x; w += 1.0;
-    if ((r = keep_new ? c->ncsdot_kept(unew, TCYM_E_MID, 1, &x) : c->ncsdot(unew, TCYM_E_MID, &x))) return r;
+    if ((r = c->ncsdot_kept(unew, TCYM_E_MID, 1, &x))) return r;
     v += x; w += 1.0;
     *out = v/w;
     return TCYM_OK;
-}
-int tcym_delta_ncs_cooling(tcym_ctx* c, int uold, int unew, double* out) {
-    if (!c || !out) return TCYM_E_ARG;
-    return delta_ncs(c, uold, unew, false, false, out);
 }
 int tcym_cooling_start(tcym_ctx* c, int u_hot, int nsteps, double dtau) {
     if (!c) return TCYM_E_ARG;
@@ -702,26 +716,14 @@ int tcym_cooling_start(tcym_ctx* c, int u_hot, int nsteps, double dtau) {
 }
 int tcym_cooling_update(tcym_ctx* c, int u_hot, int nsteps, double dtau, int measure, double* dncs) {
     if (!c) return TCYM_E_ARG;
-    // U_OLD is the previous call's U_NEW: its field strength, formed then for
-    // the last E.B of the Simpson rule, serves this call's E.B at U_OLD (one
-    // B of the three per measurement saved; the sum is the same to the bit)
-    // unless anything has accessed U_OLD since
-    const bool kept = c->bcool_ok && c->bcool_ver == c->uver[TCYM_U_OLD] && c->bcool_lev == c->ulev[TCYM_U_OLD];
-    c->bcool_ok = 0;
     int r = tcym_cool(c, u_hot, TCYM_U_NEW, nsteps, dtau);
     if (r) return r;
     if (measure) {
         double d;
-        if ((r = delta_ncs(c, TCYM_U_OLD, TCYM_U_NEW, kept, true, &d))) return r;
+        if ((r = tcym_delta_ncs_cooling(c, TCYM_U_OLD, TCYM_U_NEW, &d))) return r;
         if (dncs) *dncs = d;
     }
-    if ((r = tcym_copy_links(c, TCYM_U_OLD, TCYM_U_NEW))) return r;
-    if (measure) {
-        c->bcool_ok = 1;
-        c->bcool_ver = c->uver[TCYM_U_OLD];
-        c->bcool_lev = c->ulev[TCYM_U_OLD];
-    }
-    return TCYM_OK;
+    return tcym_copy_links(c, TCYM_U_OLD, TCYM_U_NEW);
 }
 int tcym_vacuum_ncs(tcym_ctx* c, int u, double* out) {
     if (!c || !out) return TCYM_E_ARG;
