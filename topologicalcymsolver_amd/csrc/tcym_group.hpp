@@ -296,16 +296,6 @@ template <typename T> struct SU3 {
     // (ACC false: o[a] = c ReTr(i t^a A B))
     template <bool ACC = true>
     __device__ static __forceinline__ void proj_mul_acc(T c, const Mat& A, const Mat& B, T* o) {
-        auto ent = [&](int row, int col, T& re, T& im) {
-            T sr = T(0), si = T(0);
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int ia = row + 3*k, ib = k + 3*col;
-                sr = fmat(A.r[ia], B.r[ib], sr); sr = fmat(-A.i[ia], B.i[ib], sr);
-                si = fmat(A.r[ia], B.i[ib], si); si = fmat(A.i[ia], B.r[ib], si);
-            }
-            re = sr; im = si;
-        };
         auto dim = [&](int d) -> T {
             T si = T(0);
 #pragma unroll
@@ -315,14 +305,27 @@ template <typename T> struct SU3 {
             }
             return si;
         };
-        T r10, i10, r01, i01, r20, i20, r02, i02, r21, i21, r12, i12;
-        ent(1, 0, r10, i10); ent(0, 1, r01, i01);
-        ent(2, 0, r20, i20); ent(0, 2, r02, i02);
-        ent(2, 1, r21, i21); ent(1, 2, r12, i12);
+        // for an off-diagonal pair (r, c) the projection reads only
+        // Re X_rc - Re X_cr and Im X_rc + Im X_cr (X = A B): one multiply-add
+        // chain each, over the twelve products of both entries
+        auto pair = [&](int r, int c, T& dre, T& sim) {
+            T a = T(0), b = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int ir = r + 3*k, jc = k + 3*c, ic = c + 3*k, jr = k + 3*r;
+                a = fmat(A.r[ir], B.r[jc], a); a = fmat(-A.i[ir], B.i[jc], a);
+                a = fmat(-A.r[ic], B.r[jr], a); a = fmat(A.i[ic], B.i[jr], a);
+                b = fmat(A.r[ir], B.i[jc], b); b = fmat(A.i[ir], B.r[jc], b);
+                b = fmat(A.r[ic], B.i[jr], b); b = fmat(A.i[ic], B.r[jr], b);
+            }
+            dre = a; sim = b;
+        };
+        T d01, s01, d02, s02, d12, s12;
+        pair(0, 1, d01, s01); pair(0, 2, d02, s02); pair(1, 2, d12, s12);
         const T i00 = dim(0), i11 = dim(1), i22 = dim(2);
         const T h = T(-0.5)*c, s3 = T(0.57735026918962576450914878050196);
-        const T p[8] = {h*(i01 + i10), h*(r01 - r10), h*(i00 - i11), h*(i02 + i20),
-                        h*(r02 - r20), h*(i12 + i21), h*(r12 - r21), h*(i00 + i11 - T(2)*i22)*s3};
+        const T p[8] = {h*s01, h*d01, h*(i00 - i11), h*s02,
+                        h*d02, h*s12, h*d12, h*(i00 + i11 - T(2)*i22)*s3};
 #pragma unroll
         for (int a = 0; a < 8; a++) o[a] = ACC ? o[a] + p[a] : p[a];
     }
