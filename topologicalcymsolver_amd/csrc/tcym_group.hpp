@@ -112,6 +112,13 @@ template <typename T> struct SU2 {
         for (int k = 0; k < 4; k++) C.v[k] = a*A.v[k] + b*B.v[k];
         return C;
     }
+    // A + b B
+    __device__ static __forceinline__ Mat lin1(const Mat& A, T b, const Mat& B) {
+        Mat C;
+#pragma unroll
+        for (int k = 0; k < 4; k++) C.v[k] = fmat(b, B.v[k], A.v[k]);
+        return C;
+    }
     __device__ static __forceinline__ T retr_1m(const Mat& M) { return T(2)*(T(1) - M.v[3]); }
 
     // exp(i c al^a sigma_a/2) = cos h + i (c sin(h)/(2h)) al.sigma, h = |c al|/2.
@@ -335,6 +342,13 @@ template <typename T> struct SU3 {
         Mat C;
 #pragma unroll
         for (int k = 0; k < 9; k++) { C.r[k] = a*A.r[k] + b*B.r[k]; C.i[k] = a*A.i[k] + b*B.i[k]; }
+        return C;
+    }
+    // A + b B
+    __device__ static __forceinline__ Mat lin1(const Mat& A, T b, const Mat& B) {
+        Mat C;
+#pragma unroll
+        for (int k = 0; k < 9; k++) { C.r[k] = fmat(b, B.r[k], A.r[k]); C.i[k] = fmat(b, B.i[k], A.i[k]); }
         return C;
     }
 
