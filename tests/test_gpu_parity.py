@@ -197,6 +197,39 @@ def test_gauss_restore_reports_iteration_cap():
     assert ei.value.code == T.E_NOCONV
 
 
+def test_geodesic_large_and_degenerate_angles_match_oracle():
+    """U_new = exp(i A) U_old per link with the eigenvalues of A spread over
+    (-pi + 0.1, pi - 0.1), and with doubly degenerate spectra (A along
+    lambda_8 or lambda_3 with equal phases): the log's Cayley shift and its
+    Jacobi rotations (hardware reciprocal estimates with Newton steps) against
+    the oracle's eigen-decomposition."""
+    import scipy.linalg as sla
+    nc, N = 3, (4, 4, 4)
+    lat, gpu = pair(nc, N, 41, 0.5, 0.0)
+    Uold = lat.U.copy()
+    rng = np.random.default_rng(43)
+    t = O.generators(3)
+    nl = Uold.size // 18
+    Unew = np.empty_like(Uold)
+    for k in range(nl):
+        if k % 4 == 0:
+            a = np.zeros(8)
+            a[7 if k % 8 == 0 else 2] = rng.uniform(-2.5, 2.5)
+        else:
+            while True:
+                a = rng.normal(0, 1.5, 8)
+                if np.max(np.abs(np.linalg.eigvalsh(O.alg_matrix(3, a)))) < np.pi - 0.1:
+                    break
+        W = sla.expm(1j * O.alg_matrix(3, a))
+        Unew[18*k:18*k + 18] = O.from_matrix(3, W @ O.to_matrix(3, Uold[18*k:18*k + 18]))
+    gpu.set_links(Uold, T.U_OLD)
+    gpu.set_links(Unew, T.U_NEW)
+    gpu.geodesic_interpolation(0.5)
+    Um, Em = lat.geodesic(Uold, Unew, 0.5)
+    np.testing.assert_allclose(gpu.get_links(T.U_MID), Um, atol=1e-11)
+    np.testing.assert_allclose(gpu.get_efield(T.E_MID), Em, atol=1e-11)
+
+
 @pytest.mark.parametrize("nc,N", CASES)
 def test_geodesic_and_cooling_delta_ncs_match_oracle(nc, N):
     lat, gpu = pair(nc, N, 16, 0.5, 0.0)
