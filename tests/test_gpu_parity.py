@@ -38,14 +38,17 @@ def test_upload_download_round_trip(nc, N):
     assert not gpu.get_efield(T.E_AUX).any()
 
 
+@pytest.mark.parametrize("sigma", [0.2, 1.5])
 @pytest.mark.parametrize("nc,N", CASES)
-def test_random_initial_conditions_match_oracle(nc, N):
+def test_random_initial_conditions_match_oracle(nc, N, sigma):
+    # sigma 1.5 takes the SU(3) exponential through its scaling and squaring
+    # (c1 = tr(Q^2)/2 well above 1/4)
     lat = O.Lattice(nc, N)
-    lat.random(1234, 0.2, 0.3, restore=False)
+    lat.random(1234, sigma, 0.3, restore=False)
     gpu = T.Solver(nc, N)
-    gpu.random_links(1234, 0.2)
+    gpu.random_links(1234, sigma)
     gpu.random_efield(1234, 0.3)
-    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-14)
+    np.testing.assert_allclose(gpu.get_links(), lat.U, atol=1e-14 if sigma < 1 else 1e-13)
     np.testing.assert_allclose(gpu.get_efield(), lat.E, atol=1e-14)
 
 
