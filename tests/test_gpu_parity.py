@@ -272,6 +272,24 @@ def test_cooling_method_update_matches_oracle(nc, N):
 
 
 @pytest.mark.parametrize("nc", [2, 3])
+def test_fp32_link_update_at_large_angles_tracks_fp64(nc):
+    # Dynamics::UpdateGaugeLinks with E large enough that the exponent's
+    # c1 = tr(Q^2)/2 exceeds 1/4 (SU(3): scaling and squaring in FP32; SU(2):
+    # the sin/cos branch), FP32 against FP64 on the same fields
+    N = (6, 6, 4)
+    lat = O.Lattice(nc, N)
+    lat.random(29, 0.4, 2.5, restore=False)
+    g64 = T.Solver(nc, N, T.FP64)
+    g32 = T.Solver(nc, N, T.FP32)
+    for g in (g64, g32):
+        g.set_links(lat.U)
+        g.set_efield(lat.E)
+        g.update_gauge_links(1.0)
+    np.testing.assert_allclose(g32.get_links(), g64.get_links(), atol=3e-6)
+    assert g32.unitarity_violation()[0] < 2e-6
+
+
+@pytest.mark.parametrize("nc", [2, 3])
 def test_fp32_form_tracks_fp64(nc):
     N = (8, 8, 8)
     lat, g64 = pair(nc, N, 19, 0.3, 0.3, restore=True)
